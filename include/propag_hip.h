@@ -1,0 +1,152 @@
+/*
+ * propag_hip.h -- C ABI of the MI355X (gfx950) implementation of propag25's
+ * two GPU stages: the firelib Rothermel/BEHAVE burn pass and the arrival-time
+ * propagation.  Plain pointers and sizes only; every entry point returns a
+ * pg_status and leaves a message for pg_last_error() on failure.
+ *
+ * The layouts of GT / GeoTransform / GeoReference / Settings are identical
+ * to the reference's #[repr(C)] types so a caller can pass the same bytes:
+ *   GT, GeoTransform, GeoReference   crates/geometry/src/lib.rs:20-29,128-133,180-187
+ *   Settings                         crates/propag/src/propag.rs:38-43
+ *
+ * Device pointers are HIP device allocations on the current device; streams
+ * are hipStream_t passed as void* (NULL = the default stream).
+ */
+#ifndef PROPAG_HIP_H
+#define PROPAG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct GT {
+  float x0, dx, rx, y0, ry, dy;
+} GT;
+
+typedef struct GeoTransform {
+  GT gt;
+  GT inv;
+} GeoTransform;
+
+typedef struct GeoReference {
+  uint32_t width;
+  uint32_t height;
+  uint8_t proj[1024];
+  GeoTransform transform;
+} GeoReference;
+
+typedef struct Settings {
+  GeoReference geo_ref;
+  float max_time;
+} Settings;
+
+typedef enum pg_status {
+  PG_OK = 0,
+  PG_ERR_INVALID = 1,     /* bad argument (sizes, null pointers, layout)   */
+  PG_ERR_HIP = 2,         /* a HIP runtime call failed                      */
+  PG_ERR_NO_DEVICE = 3,   /* no usable gfx950 device                        */
+  PG_ERR_NOT_CONVERGED = 4, /* the iteration cap was reached                */
+  PG_ERR_OUT_OF_MEMORY = 5, /* device allocation or admission refused       */
+  PG_ERR_TIMEOUT = 6,     /* a device-side wait exceeded its spin limit     */
+  PG_ERR_BUSY = 7         /* serving queue full / slot unavailable          */
+} pg_status;
+
+/* Propagation schedule knobs.  Zero fields take the library defaults. */
+typedef struct pg_tuning {
+  uint32_t tile_w, tile_h;      /* cells per tile (default 16 x 16)            */
+  uint32_t workgroups;          /* persistent workgroups (0 = fill the device) */
+  uint32_t max_iterations;      /* Jacobi sweep cap (0 = 64 * (W + H) + 1024)  */
+  uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep */
+  uint32_t spin_limit_log2;     /* device wait limit, 2^n polls (0 = 22)       */
+  uint32_t reserved[2];
+} pg_tuning;
+
+/* Counters of one propagation (device-side counts copied back). */
+typedef struct pg_stats {
+  uint64_t iterations;      /* Jacobi sweeps, including the final quiet one */
+  uint64_t relaxations;     /* cells whose update rule was evaluated        */
+  uint64_t improvements;    /* cell updates committed                       */
+  uint64_t dda_steps;       /* DDA cells visited by boundary walks          */
+  uint64_t tiles_visited;   /* tile visits (live tiles summed over sweeps)  */
+  double   elapsed_ms;      /* device time of the whole call (HIP events)   */
+  double   relax_ms;        /* device time of the relaxation kernel(s) only */
+} pg_stats;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Human-readable message of the last failing call on this thread. */
+const char *pg_last_error(void);
+
+/* Version string of the library build. */
+const char *pg_version(void);
+
+/* ---------------------------------------------------------------- Stage F */
+
+/* Host-only diagnostic (no GPU needed): writes the f32 catalog constants of
+ * fuel `code` (0..13) that the burn kernel uses, in this order: sigma, beta,
+ * flux_ratio, slope_k, wind_b, wind_e, wind_k, live_rx_factor,
+ * dead_rx_factor, live_area_weight, dead_area_weight, fine_dead_factor,
+ * live_ext_factor, bulk_density, residence_time, mext.  Returns the number
+ * of particles, 0 for a code without fuel, -1 for a code out of range.
+ * Exposes Catalog::STANDARD (crates/firelib/src/lib.rs:125-268). */
+int pg_catalog_f32(int code, float out[16]);
+
+/* Per-cell Rothermel/BEHAVE burn over n cells of SoA terrain (TerrainCuda,
+ * crates/firelib/src/firelib.rs:95-124: moistures as ratios, wind speed in
+ * m/s, wind azimuth / aspect in radians).  Writes speed_max (m/s),
+ * azimuth_max (rad) and eccentricity only where the fuel code is a catalog
+ * fuel with particles; the caller pre-zeroes the outputs.
+ * Replaces the kernel cuda_standard_simple_burn
+ * (crates/firelib/src/cuda.rs:57-95) and its launch in
+ * crates/propag/src/propag.rs:458-480.  All pointers are device pointers. */
+pg_status pg_burn(size_t n, const uint8_t *fuel, const float *d1hr,
+                  const float *d10hr, const float *d100hr, const float *herb,
+                  const float *wood, const float *wind_speed,
+                  const float *wind_azimuth, const float *slope,
+                  const float *aspect, float *speed_max, float *azimuth_max,
+                  float *eccentricity, void *stream);
+
+/* ---------------------------------------------------------------- Stage P */
+
+/* Boundaries, relaxation to convergence and post-pass over one raster.
+ * Replaces the kernels pre_propagate / propag / post_propagate
+ * (crates/propag/src/propag.cu:382-421) and the host loop of
+ * crates/propag/src/propag.rs:482-557.
+ *   time, refs_x, refs_y  in-out (initial state as propag.rs:433-449 builds it:
+ *                         ignited cells hold their time and point at
+ *                         themselves, other cells FLT_MAX and 0xFFFF);
+ *   boundaries            out, ceil(W*H/32) words (propag.cu:9-13 bit order).
+ * On return time holds FLT_MAX where the fire is null or the arrival is
+ * later than max_time (propag.cu:110-117).  All arrays are device pointers;
+ * the call synchronises the stream.  tuning and stats may be NULL. */
+pg_status pg_propagate(const Settings *settings, const float *speed_max,
+                       const float *azimuth_max, const float *eccentricity,
+                       float *time, uint16_t *refs_x, uint16_t *refs_y,
+                       uint32_t *boundaries, const pg_tuning *tuning,
+                       pg_stats *stats, void *stream);
+
+/* Host-buffer form of the whole device data plane of propagate()
+ * (crates/propag/src/propag.rs:375-573): uploads the SoA terrain, runs the
+ * burn pass then pg_propagate, copies time / refs / boundaries back.  The
+ * terrain columns are W*H host arrays; time/refs_x/refs_y are in-out host
+ * arrays; boundaries (out) has ceil(W*H/32) words.  fire_out, if not NULL,
+ * receives the 3*W*H burn outputs (speed_max | azimuth_max | eccentricity). */
+pg_status pg_propagate_host(const Settings *settings, const uint8_t *fuel,
+                            const float *d1hr, const float *d10hr,
+                            const float *d100hr, const float *herb,
+                            const float *wood, const float *wind_speed,
+                            const float *wind_azimuth, const float *slope,
+                            const float *aspect, float *time,
+                            uint16_t *refs_x, uint16_t *refs_y,
+                            uint32_t *boundaries, float *fire_out,
+                            const pg_tuning *tuning, pg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

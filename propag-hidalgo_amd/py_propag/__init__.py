@@ -1,0 +1,419 @@
+"""py_propag -- Python front of the MI355X implementation of propag25's GPU stages.
+
+Names follow the reference's Python package (crates/py-propag/py_propag/
+__init__.py:6-29 and crates/py-propag/src/lib.rs): ``GeoReference``,
+``Settings``, ``TimeFeature``, ``PropagResults``, ``propagate``,
+``load_terrain_data``, ``check_cuda_available`` and the exception types.
+Unlike the reference's mock (py-propag/src/lib.rs:676-691), ``propagate``
+runs the real burn + propagation on the GPU through libpropag_hip.so.  Output
+files (GeoTIFF/GPKG) and WKB rasterisation are outside this package; ignitions
+are given as raster cells.
+
+Two levels:
+  * host arrays (numpy): ``burn``, ``propagate`` -- the whole device data
+    plane of the reference's propagate() (crates/propag/src/propag.rs:375-573);
+  * device tensors (torch, cuda:N): ``burn_device``, ``propagate_device`` --
+    the C-ABI entry points pg_burn / pg_propagate on resident HBM buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from ._native import Stats, Tuning
+
+__version__ = "0.1.0"
+
+FLT_MAX = np.float32(np.finfo(np.float32).max)
+NO_REF = 0xFFFF
+TERRAIN_FIELDS = ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed",
+                  "wind_azimuth", "slope", "aspect")
+
+
+class PropagCudaError(RuntimeError):
+    """A GPU runtime failure (name kept from the reference; this is HIP)."""
+
+
+PropagGpuError = PropagCudaError
+
+
+class PropagGdalError(RuntimeError):
+    """Kept for API compatibility; raster/vector I/O is not part of this package."""
+
+
+class PropagValidationError(ValueError):
+    """Invalid arguments (shapes, ranges, layout)."""
+
+
+class PropagMPIError(RuntimeError):
+    """Kept for API compatibility; no MPI is used."""
+
+
+class PropagNotConverged(PropagCudaError):
+    """The Jacobi sweep cap was reached before a quiet sweep."""
+
+
+def _raise_for(status: int, what: str):
+    if status == 0:
+        return
+    msg = f"{what}: {_native.STATUS.get(status, status)}: {_native.last_error()}"
+    if status == 1:
+        raise PropagValidationError(msg)
+    if status == 4:
+        raise PropagNotConverged(msg)
+    raise PropagCudaError(msg)
+
+
+# ------------------------------------------------------------ geometry
+
+def _invert(gt):
+    """GT::invert (crates/geometry/src/lib.rs:86-125) in f32."""
+    x0, dx, rx, y0, ry, dy = (np.float32(v) for v in gt)
+    if rx == 0 and ry == 0 and dx != 0 and dy != 0:
+        return [-x0 / dx, np.float32(1) / dx, np.float32(0), -y0 / dy, np.float32(0),
+                np.float32(1) / dy]
+    det = dx * dy - rx * ry
+    mag = max(abs(dx), abs(rx), abs(ry), abs(dy))
+    if abs(det) <= 1e-10 * mag * mag:
+        raise PropagValidationError("geo transform is not invertible")
+    inv = np.float32(1) / det
+    return [(rx * y0 - x0 * dy) * inv, dy * inv, -rx * inv, (-dx * y0 + x0 * ry) * inv,
+            -ry * inv, dx * inv]
+
+
+class GeoReference:
+    """Raster geometry (crates/geometry/src/lib.rs:180-187).
+
+    transform = [x_origin, pixel_width, x_rotation, y_origin, y_rotation,
+    pixel_height]; a negative pixel_height is north-up."""
+
+    def __init__(self, width: int, height: int, proj: str, transform):
+        if width <= 0 or height <= 0:
+            raise PropagValidationError("Grid dimensions must be positive (width > 0, height > 0)")
+        if width > 65535 or height > 65535:
+            raise PropagValidationError("Grid dimensions too large (max 65535 per side, "
+                                        "16-bit references)")
+        pb = proj.encode()
+        if len(pb) >= 1024:
+            raise PropagValidationError("Projection string too large (max 1023 characters)")
+        t = [float(v) for v in np.asarray(transform, dtype=np.float64).ravel()]
+        if len(t) != 6:
+            raise PropagValidationError("Geo transform must have exactly 6 elements")
+        if t[1] == 0.0 or t[5] == 0.0 or not all(math.isfinite(v) for v in t):
+            raise PropagValidationError("Invalid geo transform: pixel size cannot be zero")
+        self._width, self._height, self._proj = int(width), int(height), proj
+        self._gt = [np.float32(v) for v in t]
+        self._inv = _invert(self._gt)
+
+    @classmethod
+    def south_up(cls, bbox, pixel_size, proj=""):
+        """GeoReference::south_up (crates/geometry/src/lib.rs:239-257)."""
+        (x0, y0), (x1, y1) = bbox
+        dx, dy = pixel_size
+        w = math.ceil(np.float32(np.float32(x1 - x0) / np.float32(dx)))
+        h = math.ceil(np.float32(np.float32(y1 - y0) / np.float32(dy)))
+        return cls(w, h, proj, [x0, dx, 0.0, y0, 0.0, dy])
+
+    @classmethod
+    def north_up(cls, bbox, pixel_size, proj=""):
+        """GeoReference::north_up (crates/geometry/src/lib.rs:258-276)."""
+        (x0, y0), (x1, y1) = bbox
+        dx, dy = pixel_size
+        w = math.ceil(np.float32(np.float32(x1 - x0) / np.float32(dx)))
+        h = math.ceil(np.float32(np.float32(y1 - y0) / np.float32(dy)))
+        return cls(w, h, proj, [x0, dx, 0.0, y1, 0.0, -dy])
+
+    width = property(lambda self: self._width)
+    height = property(lambda self: self._height)
+    proj = property(lambda self: self._proj)
+    transform = property(lambda self: np.array(self._gt, dtype=np.float64))
+    is_north_up = property(lambda self: self._gt[5] < 0)
+
+    def __len__(self):
+        return self._width * self._height
+
+    def to_c(self) -> _native.GeoReferenceC:
+        g = _native.GeoReferenceC()
+        g.width, g.height = self._width, self._height
+        pb = self._proj.encode()
+        C.memmove(g.proj, pb, len(pb))
+        g.transform.gt = _native.GT(*[float(v) for v in self._gt])
+        g.transform.inv = _native.GT(*[float(v) for v in self._inv])
+        return g
+
+    def __repr__(self):
+        return f"GeoReference({self._width}x{self._height}, transform={list(map(float, self._gt))})"
+
+
+class Settings:
+    """Settings{geo_ref, max_time} (crates/propag/src/propag.rs:38-43)."""
+
+    def __init__(self, geo_ref: GeoReference, max_time: float):
+        if not isinstance(geo_ref, GeoReference):
+            raise PropagValidationError("geo_ref must be a GeoReference")
+        self.geo_ref = geo_ref
+        self.max_time = max_time
+
+    @property
+    def max_time(self):
+        return self._max_time
+
+    @max_time.setter
+    def max_time(self, v):
+        v = float(v)
+        if not (v > 0 and math.isfinite(v)):
+            raise PropagValidationError("max_time must be positive and finite")
+        self._max_time = np.float32(v)
+
+    def to_c(self) -> _native.SettingsC:
+        s = _native.SettingsC()
+        s.geo_ref = self.geo_ref.to_c()
+        s.max_time = float(self._max_time)
+        return s
+
+
+@dataclass
+class TimeFeature:
+    """An ignition: a time (s) and either WKB geometry or raster cells.
+
+    Only ``cells`` (an iterable of (x, y)) is rasterised here; WKB input
+    needs the GDAL rasteriser of the reference (propag.rs:837-890)."""
+    time: float
+    geom_wkb: bytes = b""
+    cells: list = field(default_factory=list)
+
+
+@dataclass
+class PropagResults:
+    """Outputs of one propagation (crates/propag/src/propag.rs:311-321)."""
+    arrival_times: np.ndarray     # (height, width) f32, FLT_MAX = not burned
+    boundary_change: np.ndarray   # ceil(W*H/32) u32 words
+    refs_x: np.ndarray            # W*H u16
+    refs_y: np.ndarray            # W*H u16
+    geo_ref: GeoReference
+    stats: dict
+    fire: tuple | None = None     # (speed_max, azimuth_max, eccentricity) rasters
+
+    @property
+    def shape(self):
+        return self.geo_ref.height, self.geo_ref.width
+
+
+def check_cuda_available() -> bool:
+    """True when a gfx950 GPU is visible (name kept from the reference)."""
+    try:
+        import torch
+        return bool(torch.cuda.is_available()) and "gfx950" in torch.cuda.get_device_properties(0).gcnArchName
+    except Exception:
+        return False
+
+
+check_gpu_available = check_cuda_available
+
+
+def library_version() -> str:
+    return _native.load().pg_version().decode()
+
+
+def load_terrain_data(width, height, proj, transform, elevation=None, slope=None, aspect=None,
+                      fuel_model=None) -> GeoReference:
+    """Validation front of the reference (py-propag/src/lib.rs:525-577)."""
+    if width <= 0 or height <= 0:
+        raise PropagValidationError("Grid dimensions must be positive")
+    if width * height > 100_000_000:
+        raise PropagValidationError("Grid too large (limit: 100 million cells).")
+    for name, arr, lo, hi in (("Elevation", elevation, -1000.0, 9000.0),
+                              ("Slope", slope, 0.0, 90.0), ("Aspect", aspect, 0.0, 360.0),
+                              ("Fuel model", fuel_model, None, None)):
+        if arr is None:
+            continue
+        arr = np.asarray(arr)
+        if arr.shape != (height, width):
+            raise PropagValidationError(f"{name} array shape mismatch: expected ({height}, {width}), got {arr.shape}")
+        if lo is not None and (arr < lo).any():
+            raise PropagValidationError(f"{name} value below minimum {lo}")
+        if hi is not None and (arr > hi).any():
+            raise PropagValidationError(f"{name} value exceeds maximum {hi}")
+    if fuel_model is not None:
+        _validate_fuel(np.asarray(fuel_model))
+    return GeoReference(width, height, proj, transform)
+
+
+def _validate_fuel(fuel):
+    bad = np.nonzero(fuel.ravel() > 13)[0]
+    if bad.size:
+        i = int(bad[0])
+        raise PropagValidationError(f"Invalid fuel model ID {int(fuel.ravel()[i])} at index {i}. "
+                                    "Valid NFFL fuel models are 1-13.")
+
+
+def _terrain_columns(terrain: dict, n: int):
+    fuel = np.ascontiguousarray(np.asarray(terrain["fuel"], dtype=np.uint8).ravel())
+    if fuel.size != n:
+        raise PropagValidationError(f"fuel raster has {fuel.size} cells, expected {n}")
+    cols = []
+    for k in TERRAIN_FIELDS:
+        if k not in terrain:
+            raise PropagValidationError(f"terrain is missing '{k}'")
+        v = np.asarray(terrain[k], dtype=np.float32)
+        if v.size == 1:
+            v = np.full(n, v.item(), np.float32)
+        v = np.ascontiguousarray(v.ravel())
+        if v.size != n:
+            raise PropagValidationError(f"terrain '{k}' has {v.size} cells, expected {n}")
+        cols.append(v)
+    return fuel, cols
+
+
+def initial_state(width: int, height: int, ignitions):
+    """time / refs as propag.rs:433-449 builds them from ignited cells.
+    ``ignitions``: iterable of (x, y, t) or TimeFeature with ``cells``."""
+    n = width * height
+    time = np.full(n, FLT_MAX, np.float32)
+    rx = np.full(n, NO_REF, np.uint16)
+    ry = np.full(n, NO_REF, np.uint16)
+    for ign in ignitions:
+        if isinstance(ign, TimeFeature):
+            pts = [(x, y, ign.time) for x, y in ign.cells]
+        else:
+            pts = [ign]
+        for x, y, t in pts:
+            if not (0 <= x < width and 0 <= y < height):
+                raise PropagValidationError(f"ignition cell ({x}, {y}) is outside the raster")
+            i = int(x) + int(y) * width
+            time[i] = np.float32(t)
+            rx[i], ry[i] = x, y
+    return time, rx, ry
+
+
+def _tuning(tuning):
+    if tuning is None:
+        return None
+    if isinstance(tuning, Tuning):
+        return tuning
+    t = Tuning()
+    for k, v in dict(tuning).items():
+        setattr(t, k, int(v))
+    return t
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def propagate(settings: Settings, terrain: dict, ignitions=None, time=None, refs_x=None,
+              refs_y=None, tuning=None, return_fire: bool = False) -> PropagResults:
+    """Burn pass + propagation on the GPU from host rasters.
+
+    ``terrain``: dict with 'fuel' (u8) and the TerrainCuda columns
+    (TERRAIN_FIELDS; scalars broadcast).  Either ``ignitions`` or an initial
+    (time, refs_x, refs_y) state must be given."""
+    g = settings.geo_ref
+    n = len(g)
+    fuel, cols = _terrain_columns(terrain, n)
+    _validate_fuel(fuel)
+    if time is None:
+        if not ignitions:
+            raise PropagValidationError("At least one ignition element is required")
+        time, refs_x, refs_y = initial_state(g.width, g.height, ignitions)
+    time = np.ascontiguousarray(np.asarray(time, np.float32).ravel()).copy()
+    refs_x = np.ascontiguousarray(np.asarray(refs_x, np.uint16).ravel()).copy()
+    refs_y = np.ascontiguousarray(np.asarray(refs_y, np.uint16).ravel()).copy()
+    if time.size != n or refs_x.size != n or refs_y.size != n:
+        raise PropagValidationError("initial state rasters do not match the geo reference")
+    bits = np.zeros((n + 31) // 32, np.uint32)
+    fire = np.zeros(3 * n, np.float32) if return_fire else None
+    st = Stats()
+    tn = _tuning(tuning)
+    lib = _native.load()
+    sc = settings.to_c()
+    rc = lib.pg_propagate_host(C.byref(sc), _ptr(fuel), *[_ptr(c) for c in cols], _ptr(time),
+                               _ptr(refs_x), _ptr(refs_y), _ptr(bits),
+                               _ptr(fire) if fire is not None else None,
+                               C.byref(tn) if tn is not None else None, C.byref(st))
+    _raise_for(rc, "pg_propagate_host")
+    fr = tuple(fire.reshape(3, n)) if fire is not None else None
+    return PropagResults(time.reshape(g.height, g.width), bits, refs_x, refs_y, g, st.as_dict(), fr)
+
+
+# --------------------------------------------------------- device level
+
+def _dptr(t):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+        raise PropagValidationError("expected a contiguous GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return C.c_void_p(getattr(stream, "cuda_stream", stream))
+
+
+def burn_device(fuel, d1hr, d10hr, d100hr, herb, wood, wind_speed, wind_azimuth, slope, aspect,
+                speed_max, azimuth_max, eccentricity, stream=None):
+    """pg_burn on resident tensors (u8 fuel, f32 columns, f32 outputs the
+    caller pre-zeroed).  Asynchronous on ``stream`` (torch's current stream
+    by default)."""
+    n = fuel.numel()
+    for t in (d1hr, d10hr, d100hr, herb, wood, wind_speed, wind_azimuth, slope, aspect,
+              speed_max, azimuth_max, eccentricity):
+        if t.numel() != n:
+            raise PropagValidationError("all burn rasters must have the same number of cells")
+    rc = _native.load().pg_burn(n, _dptr(fuel), _dptr(d1hr), _dptr(d10hr), _dptr(d100hr),
+                                _dptr(herb), _dptr(wood), _dptr(wind_speed), _dptr(wind_azimuth),
+                                _dptr(slope), _dptr(aspect), _dptr(speed_max), _dptr(azimuth_max),
+                                _dptr(eccentricity), _stream_ptr(stream))
+    _raise_for(rc, "pg_burn")
+
+
+def propagate_device(settings: Settings, speed_max, azimuth_max, eccentricity, time, refs_x,
+                     refs_y, boundaries, tuning=None, stream=None) -> dict:
+    """pg_propagate on resident tensors: time (f32) and refs (16-bit, e.g.
+    torch.int16 holding the u16 bits) are in-out, boundaries (32-bit words)
+    out.  Synchronises ``stream``; returns the stats dict."""
+    n = len(settings.geo_ref)
+    for t, want in ((speed_max, n), (azimuth_max, n), (eccentricity, n), (time, n), (refs_x, n),
+                    (refs_y, n), (boundaries, (n + 31) // 32)):
+        if t.numel() != want:
+            raise PropagValidationError("raster sizes do not match the geo reference")
+    if refs_x.element_size() != 2 or refs_y.element_size() != 2 or boundaries.element_size() != 4:
+        raise PropagValidationError("refs must be 16-bit and boundaries 32-bit tensors")
+    st = Stats()
+    tn = _tuning(tuning)
+    sc = settings.to_c()
+    rc = _native.load().pg_propagate(C.byref(sc), _dptr(speed_max), _dptr(azimuth_max),
+                                     _dptr(eccentricity), _dptr(time), _dptr(refs_x),
+                                     _dptr(refs_y), _dptr(boundaries),
+                                     C.byref(tn) if tn is not None else None, C.byref(st),
+                                     _stream_ptr(stream))
+    _raise_for(rc, "pg_propagate")
+    return st.as_dict()
+
+
+def burn(terrain: dict):
+    """Stage F on the GPU from host rasters; returns numpy (speed_max,
+    azimuth_max, eccentricity), zero where the fuel is not a catalog fuel."""
+    import torch
+    fuel, cols = _terrain_columns(terrain, np.asarray(terrain["fuel"]).size)
+    dev = torch.device("cuda")
+    tf = torch.from_numpy(fuel).to(dev)
+    tc = [torch.from_numpy(c).to(dev) for c in cols]
+    out = [torch.zeros(fuel.size, dtype=torch.float32, device=dev) for _ in range(3)]
+    burn_device(tf, *tc, *out)
+    torch.cuda.synchronize()
+    return tuple(o.cpu().numpy() for o in out)
+
+
+__all__ = [
+    "Settings", "GeoReference", "TimeFeature", "PropagResults", "propagate", "load_terrain_data",
+    "check_cuda_available", "check_gpu_available", "PropagCudaError", "PropagGpuError",
+    "PropagGdalError", "PropagValidationError", "PropagMPIError", "PropagNotConverged", "burn",
+    "burn_device", "propagate_device", "initial_state", "Tuning", "Stats", "FLT_MAX", "NO_REF",
+]

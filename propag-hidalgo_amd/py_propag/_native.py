@@ -1,0 +1,107 @@
+"""ctypes binding of libpropag_hip.so (include/propag_hip.h).
+
+The library must be built (``__graft_entry__.build()``); there is no CPU
+fallback: loading fails loudly when the .so is missing.  torch is imported
+before the library so that both share one HIP runtime (torch's bundled
+libamdhip64.so.7 satisfies the library's dependency by SONAME).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libpropag_hip.so")
+
+EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propagate",
+           "pg_propagate_host")
+
+
+class GT(C.Structure):
+    _fields_ = [("x0", C.c_float), ("dx", C.c_float), ("rx", C.c_float),
+                ("y0", C.c_float), ("ry", C.c_float), ("dy", C.c_float)]
+
+
+class GeoTransform(C.Structure):
+    _fields_ = [("gt", GT), ("inv", GT)]
+
+
+class GeoReferenceC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32),
+                ("proj", C.c_uint8 * 1024), ("transform", GeoTransform)]
+
+
+class SettingsC(C.Structure):
+    _fields_ = [("geo_ref", GeoReferenceC), ("max_time", C.c_float)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("workgroups", C.c_uint32),
+                ("max_iterations", C.c_uint32), ("mode", C.c_uint32),
+                ("spin_limit_log2", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("relaxations", C.c_uint64),
+                ("improvements", C.c_uint64), ("dda_steps", C.c_uint64),
+                ("tiles_visited", C.c_uint64), ("elapsed_ms", C.c_double),
+                ("relax_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+STATUS = {0: "PG_OK", 1: "PG_ERR_INVALID", 2: "PG_ERR_HIP", 3: "PG_ERR_NO_DEVICE",
+          4: "PG_ERR_NOT_CONVERGED", 5: "PG_ERR_OUT_OF_MEMORY", 6: "PG_ERR_TIMEOUT",
+          7: "PG_ERR_BUSY"}
+
+_lib = None
+
+
+def load(require_torch: bool = True):
+    """Loads the library once.  Raises OSError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if require_torch:
+        try:
+            import torch  # noqa: F401  (share torch's HIP runtime)
+        except ImportError:
+            pass
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                      "(there is no CPU fallback for the propagation path)")
+    lib = C.CDLL(LIB_PATH)
+    vp, sz = C.c_void_p, C.c_size_t
+    lib.pg_last_error.restype = C.c_char_p
+    lib.pg_version.restype = C.c_char_p
+    lib.pg_catalog_f32.restype = C.c_int
+    lib.pg_catalog_f32.argtypes = [C.c_int, C.c_void_p]
+    lib.pg_burn.restype = C.c_int
+    lib.pg_burn.argtypes = [sz] + [vp] * 14
+    lib.pg_propagate.restype = C.c_int
+    lib.pg_propagate.argtypes = [C.POINTER(SettingsC)] + [vp] * 7 + [C.POINTER(Tuning),
+                                                                     C.POINTER(Stats), vp]
+    lib.pg_propagate_host.restype = C.c_int
+    lib.pg_propagate_host.argtypes = [C.POINTER(SettingsC)] + [vp] * 15 + [C.POINTER(Tuning),
+                                                                           C.POINTER(Stats)]
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names from include/propag_hip.h that the loaded library exports."""
+    lib = load(require_torch=False)
+    found = []
+    for name in EXPORTS:
+        try:
+            getattr(lib, name)
+            found.append(name)
+        except AttributeError:
+            pass
+    return found
+
+
+def last_error() -> str:
+    msg = load().pg_last_error()
+    return msg.decode() if msg else ""

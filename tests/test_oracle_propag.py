@@ -1,0 +1,305 @@
+"""Pins the Stage P oracle (oracle/propag_oracle.c) without retyping it.
+
+* DDA: the reference's own line_to known answers
+  (crates/geometry/src/lib.rs:508-649), structural properties of every walk,
+  and agreement with the exact rational DDA away from float ties.
+* time_from: the closed algebraic form against an FP64 evaluation of the
+  reference's atan2/cos formula (propag.cu:269-294), both orientations and
+  non-square cells.
+* Uniform fuel: arrival times equal the closed-form ellipse from the
+  ignition; the reference CLI scenario (README.md:92-113) burns the same
+  number of cells as the closed form, within 1% of the reference's printed
+  447,822 (its older tiled build rounds ties differently).
+* No wind, flat: the burned set is the disk of radius speed0 * max_time.
+* Boundaries against a vectorised numpy restatement of propag.cu:296-375.
+* Fixed-point invariants, schedule (thread count) independence, edge cases.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from py_propag import synth
+
+FLT_MAX = oracle.FLT_MAX
+
+
+# ------------------------------------------------------------------ DDA
+
+@pytest.mark.parametrize("to,expected", [
+    ((0, 2), [(0, 0), (0, 1), (0, 2)]),
+    ((0, 1), [(0, 0), (0, 1)]),
+    ((0, -2), [(0, 0), (0, -1), (0, -2)]),
+    ((0, -1), [(0, 0), (0, -1)]),
+    ((2, 0), [(0, 0), (1, 0), (2, 0)]),
+    ((-2, 0), [(0, 0), (-1, 0), (-2, 0)]),
+    ((-2, -2), [(0, 0), (-1, -1), (-2, -2)]),
+    ((-2, 2), [(0, 0), (-1, 1), (-2, 2)]),
+    ((2, -2), [(0, 0), (1, -1), (2, -2)]),
+    ((1, -1), [(0, 0), (1, -1)]),
+    ((2, 2), [(0, 0), (1, 1), (2, 2)]),
+    ((0, 0), [(0, 0)]),
+])
+def test_dda_reference_cases(oracle_lib, to, expected):
+    assert oracle.dda(0, 0, *to) == expected
+
+
+def test_dda_special_case_propag(oracle_lib):
+    assert oracle.dda(499, 99, 500, 100)[1] == (500, 100)
+
+
+def test_dda_second_cell_is_neighbor_in_direction(oracle_lib):
+    # geometry/src/lib.rs:509-518 over a sub-sampled cross of offsets
+    for x in range(-300, 301, 7):
+        for y in range(-300, 301, 11):
+            if x == 0 or y == 0:
+                continue
+            if abs(x) == abs(y):
+                want = (int(np.sign(x)), int(np.sign(y)))
+            elif abs(x) > abs(y):
+                want = (int(np.sign(x)), 0)
+            else:
+                want = (0, int(np.sign(y)))
+            assert oracle.dda(0, 0, x, y, max_out=2)[1] == want
+
+
+def _exact_dda(dx, dy):
+    """The DDA in exact rational arithmetic: step x when (i+1)/|dx| <
+    (j+1)/|dy|, y when greater, both when equal."""
+    sx, sy = int(np.sign(dx)), int(np.sign(dy))
+    ax, ay = abs(dx), abs(dy)
+    x = y = i = j = 0
+    out = [(0, 0)]
+    while (x, y) != (dx, dy):
+        lhs = (i + 1) * ay if ax else math.inf
+        rhs = (j + 1) * ax if ay else math.inf
+        if ax and ay and lhs == rhs:
+            x += sx; y += sy; i += 1; j += 1
+        elif (ay == 0) or (ax and lhs < rhs):
+            x += sx; i += 1
+        else:
+            y += sy; j += 1
+        out.append((x, y))
+    return out
+
+
+def test_dda_structure_and_exact_rule(oracle_lib):
+    """The f32 walk is the exact rational walk except at exact ties
+    ((i+1)/|dx| == (j+1)/|dy|, e.g. the last step of most lines), where f32
+    accumulation of 1/|d| usually misses the 1e-9 tie window and the walk
+    takes the two orthogonal steps around the corner instead of the
+    diagonal one.  For |d| <= 150 distinct fractions differ by far more than
+    the accumulated rounding, so no other decision can differ."""
+    r = np.random.default_rng(5)
+    corners = 0
+    for _ in range(400):
+        dx, dy = (int(v) for v in r.integers(-150, 151, 2))
+        if dx == 0 and dy == 0:
+            continue
+        p = oracle.dda(0, 0, dx, dy)
+        assert p[0] == (0, 0) and p[-1] == (dx, dy)
+        for (x0, y0), (x1, y1) in zip(p, p[1:]):
+            assert max(abs(x1 - x0), abs(y1 - y0)) == 1          # 8-connected
+            assert (x1 - x0) * np.sign(dx) >= 0 and (y1 - y0) * np.sign(dy) >= 0  # monotone
+        exact = _exact_dda(dx, dy)
+        ex = set(exact)
+        assert ex <= set(p)
+        assert [c for c in p if c in ex] == exact
+        for k, c in enumerate(p):
+            if c in ex:
+                continue
+            corners += 1
+            (ax, ay), (bx, by) = p[k - 1], p[k + 1]
+            assert abs(ax - bx) == 1 and abs(ay - by) == 1       # replaces a diagonal step
+    assert corners > 0
+
+
+# ------------------------------------------------------------ time_from
+
+def _fp64_time_from(t0, fx, fy, tx, ty, s, az, e, gdx, gdy):
+    north_up = gdy < 0
+    a = math.atan2(tx - fx, (fy - ty) if north_up else (ty - fy))
+    factor = (1 - e) / (1 - e * math.cos(abs(a - az)))
+    d = math.hypot((fx - tx) * gdx, (fy - ty) * gdy)
+    return t0 + d / (s * factor)
+
+
+@pytest.mark.parametrize("gdx,gdy", [(5.0, 5.0), (5.0, -5.0), (5.0, 10.0), (2.0, -7.0)])
+def test_time_from_matches_fp64_reference_formula(oracle_lib, gdx, gdy):
+    g = oracle.grid(4096, 4096, gdx, gdy, 1e30)
+    r = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(3000):
+        fx, fy, tx, ty = (int(v) for v in r.integers(0, 600, 4))
+        if (fx, fy) == (tx, ty):
+            continue
+        s = float(np.float32(r.uniform(0.01, 3.0)))
+        az = float(np.float32(r.uniform(0, 2 * np.pi)))
+        e = float(np.float32(r.uniform(0, 0.98)))
+        t0 = float(np.float32(r.uniform(0, 5000)))
+        want = _fp64_time_from(t0, fx, fy, tx, ty, s, az, e, gdx, gdy)
+        for formula in (0, 1):
+            got = oracle.time_from(g, t0, fx, fy, tx, ty, s, az, e, formula)
+            worst = max(worst, abs(got - want) / (want - t0))
+    assert worst < 2e-5
+
+
+def test_sincos_matches_libm(oracle_lib):
+    for x in np.linspace(-20, 20, 2001):
+        s, c = oracle.sincos(float(x))
+        assert abs(s - math.sin(x)) < 4e-16 and abs(c - math.cos(x)) < 4e-16
+
+
+# -------------------------------------------------------- propagation
+
+def _run(sc, **kw):
+    s, a, e = oracle.burn_f32(sc["terrain"])
+    g = oracle.grid(sc["width"], sc["height"], sc["dx"], sc["dy"], sc["max_time"])
+    res = oracle.propagate(g, s, a, e, sc["time"], sc["refs_x"], sc["refs_y"], **kw)
+    return res, (s, a, e), g
+
+
+def _closed_form(sc, fire):
+    s, a, e = (float(v[0]) for v in fire)
+    W, H = sc["width"], sc["height"]
+    (ix, iy, t0), = sc["ignitions"]
+    yy, xx = np.mgrid[0:H, 0:W]
+    dxc, dyc = xx - ix, yy - iy
+    north_up = sc["dy"] < 0
+    az = np.arctan2(dxc, -dyc if north_up else dyc)
+    travel = np.hypot(dxc * sc["dx"], dyc * sc["dy"]) * (1 - e * np.cos(az - a)) / (s * (1 - e))
+    return (t0 + travel).ravel()
+
+
+def test_readme_known_answer_scenario(oracle_lib):
+    sc = synth.uniform_scenario(side=1024)  # main.rs:11-57
+    res, fire, _ = _run(sc, nthreads=8)
+    t = res["time"]
+    burned = t < FLT_MAX
+    closed = _closed_form(sc, fire)
+    assert int(burned.sum()) == int((closed <= 36000.0).sum()) == 450599
+    assert abs(int(burned.sum()) - 447822) / 447822 < 0.01  # README.md:112
+    assert abs(float(t[burned].max()) - 35999.9) < 1.0        # README.md:110
+    assert float(t[burned].min()) == 0.0
+    rel = np.abs(t[burned] - closed[burned]) / np.maximum(closed[burned], 1.0)
+    assert rel.max() < 1e-5
+
+
+def test_no_wind_flat_is_a_disk(oracle_lib):
+    sc = synth.uniform_scenario(side=201, wind=0.0, ignition=(100, 100), max_time=3000.0)
+    res, fire, _ = _run(sc)
+    speed0 = float(fire[0][0])
+    yy, xx = np.mgrid[0:201, 0:201]
+    d = np.hypot(xx - 100, yy - 100).ravel() * 5.0
+    want = d / speed0
+    t = res["time"]
+    burned = t < FLT_MAX
+    clear = np.abs(want - 3000.0) > 1.0  # away from the max_time rim
+    assert np.array_equal(burned[clear], (want <= 3000.0)[clear])
+    assert np.allclose(t[burned], want[burned], rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("gdx,gdy,waz", [(5.0, -5.0, 0.0), (5.0, 10.0, 0.7), (5.0, -10.0, 2.2)])
+def test_uniform_fire_orientation_and_cells(oracle_lib, gdx, gdy, waz):
+    sc = synth.uniform_scenario(side=161, ignition=(80, 80), wind=4.0, wind_az=waz,
+                                max_time=4000.0)
+    sc["dx"], sc["dy"] = gdx, gdy
+    res, fire, _ = _run(sc)
+    closed = _closed_form(sc, fire)
+    t = res["time"]
+    burned = t < FLT_MAX
+    clear = np.abs(closed - 4000.0) > 5.0
+    rel = (t[burned] - closed[burned]) / np.maximum(closed[burned], 1.0)
+    if abs(gdx) == abs(gdy):
+        # square cells: the straight line from the ignition is optimal
+        assert np.array_equal(burned[clear], (closed <= 4000.0)[clear])
+        assert np.abs(rel).max() < 1e-5
+    else:
+        # the reference takes the azimuth in cell-index space (propag.cu:274-280)
+        # while distances are metric, so with non-square cells travel time is
+        # not convex in the offset and re-anchored paths can arrive earlier:
+        # the closed form is an upper bound that is reached up to rounding.
+        assert rel.max() < 1e-5
+        assert np.all(burned[clear & (closed <= 4000.0)])
+
+
+def _numpy_boundaries(W, H, s, a, e):
+    s, a, e = (v.reshape(H, W) for v in (s, a, e))
+    out = np.zeros((H, W), bool)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx == 0 and dy == 0:
+                continue
+            ys = slice(max(0, -dy), H - max(0, dy))
+            yn = slice(max(0, dy), H - max(0, -dy))
+            xs = slice(max(0, -dx), W - max(0, dx))
+            xn = slice(max(0, dx), W - max(0, -dx))
+            sim = ((np.abs(s[yn, xn] - s[ys, xs]).astype(np.float64) < 0.1)
+                   & (np.abs(a[yn, xn] - a[ys, xs]).astype(np.float64) < 5.0 * 2 * np.pi / 360.0)
+                   & (np.abs(e[yn, xn] - e[ys, xs]).astype(np.float64) < 0.05))
+            out[ys, xs] |= ~sim & (s[ys, xs] < s[yn, xn])
+    flat = out.ravel()
+    n = flat.size
+    words = np.zeros((n + 31) // 32, np.uint64)
+    idx = np.nonzero(flat)[0]
+    np.bitwise_or.at(words, idx // 32, (np.uint64(1) << (idx % 32).astype(np.uint64)))
+    return words.astype(np.uint32)
+
+
+def test_boundaries_match_numpy(oracle_lib):
+    sc = synth.single_scenario(seed=4, side=96)
+    s, a, e = oracle.burn_f32(sc["terrain"])
+    g = oracle.grid(96, 96, 5.0, 5.0, 36000.0)
+    got = oracle.boundaries(g, s, a, e)
+    assert np.array_equal(got, _numpy_boundaries(96, 96, s, a, e))
+    assert got.any()
+
+
+def test_fixed_point_invariants_and_thread_independence(oracle_lib):
+    sc = synth.single_scenario(seed=7, side=128)
+    r1, fire, _ = _run(sc, nthreads=1)
+    r4, _, _ = _run(sc, nthreads=4)
+    for k in ("time", "refs_x", "refs_y", "boundaries"):
+        assert np.array_equal(r1[k], r4[k])
+    assert r1["sweeps"] == r4["sweeps"] and r1["improvements"] == r4["improvements"]
+    t, rx, ry = r1["time"], r1["refs_x"].astype(np.int64), r1["refs_y"].astype(np.int64)
+    burned = np.nonzero(t < FLT_MAX)[0]
+    assert burned.size > 100
+    assert np.all(rx[burned] < 128) and np.all(ry[burned] < 128)
+    ref_t = t[rx[burned] + 128 * ry[burned]]
+    ok = ref_t < FLT_MAX
+    assert np.all(ref_t[ok] <= t[burned][ok])
+    # arrival times are within the configured horizon
+    assert t[burned].max() <= 36000.0
+
+
+def test_formula_choice_changes_ties_not_the_burned_mask(oracle_lib):
+    sc = synth.single_scenario(seed=7, side=128)
+    r0, _, _ = _run(sc, formula=0)
+    r1, _, _ = _run(sc, formula=1)
+    m0, m1 = r0["time"] < FLT_MAX, r1["time"] < FLT_MAX
+    near = np.abs(r0["time"] - 36000.0) < 60.0
+    assert np.array_equal(m0[~near], m1[~near])
+
+
+def test_edge_cases(oracle_lib):
+    # ignition on a NoFuel cell: nothing spreads, post-pass clears it
+    sc = synth.uniform_scenario(side=32, fuel=0, ignition=(10, 10))
+    res, _, _ = _run(sc)
+    assert np.all(res["time"] == FLT_MAX) and res["improvements"] == 0
+    # no ignition at all
+    sc = synth.uniform_scenario(side=16, ignition=(3, 3))
+    sc["time"][:] = FLT_MAX
+    sc["refs_x"][:] = 0xFFFF
+    sc["refs_y"][:] = 0xFFFF
+    res, _, _ = _run(sc)
+    assert np.all(res["time"] == FLT_MAX) and res["sweeps"] == 0
+    # single cell raster
+    sc = synth.uniform_scenario(side=1, ignition=(0, 0))
+    res, _, _ = _run(sc)
+    assert res["time"][0] == 0.0
+    # max_time shorter than one cell of travel: only the ignition burns
+    sc = synth.uniform_scenario(side=9, ignition=(4, 4), max_time=1.0)
+    res, _, _ = _run(sc)
+    assert (res["time"] < FLT_MAX).sum() == 1
