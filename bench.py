@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", type=int, default=0, help="0 persistent kernel, 1 one launch per sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-phases", action="store_true",
+                    help="time the sweep phases inside the kernel (slower; diagnostics only)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (rocprof counter scaling)")
     args = ap.parse_args()
@@ -86,7 +88,7 @@ def main():
     rx_buf = torch.empty_like(rx_init)
     ry_buf = torch.empty_like(ry_init)
     bits = torch.zeros((n + 31) // 32, dtype=torch.int32, device=dev)
-    tuning = {"mode": args.mode}
+    tuning = {"mode": args.mode, "profile": 1 if args.profile_phases else 0}
 
     def step():
         t_buf.copy_(t_init)
@@ -121,7 +123,10 @@ def main():
     alg_bytes = 24.0 * relax + 8.0 * impr
     achieved = alg_bytes / (relax_ms.mean() * 1e-3) / 1e9
     log(f"propagation: {ms_per_step:.3f} ms/scenario, relax kernel {relax_ms.mean():.3f} ms, "
-        f"{stats[-1]['iterations']} sweeps, {relax} relaxations, {burned} burned")
+        f"{stats[-1]['iterations']} sweeps, {relax} relaxations, {burned} burned; per workgroup "
+        f"work {stats[-1]['work_ms']:.3f} ms, barrier wait {stats[-1]['wait_ms']:.3f} ms, "
+        f"{stats[-1]['workgroups']} workgroups, LDS bitset {stats[-1]['lds_bitset']}, critical path "
+        f"{stats[-1]['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) {stats[-1]['phase_ms']}")
 
     # ---------------------------------------------------- Rothermel field
     field = synth.rothermel_field(seed=0, side=1024)
@@ -201,7 +206,12 @@ def main():
         "propagation": {"sweeps": stats[-1]["iterations"], "relaxations": relax,
                         "improvements": impr, "dda_steps": stats[-1]["dda_steps"],
                         "tiles_visited": stats[-1]["tiles_visited"], "burned_cells": burned,
-                        "elapsed_ms": float(np.mean([s["elapsed_ms"] for s in stats]))},
+                        "elapsed_ms": float(np.mean([s["elapsed_ms"] for s in stats])),
+                        "work_ms": float(np.mean([s["work_ms"] for s in stats])),
+                        "wait_ms": float(np.mean([s["wait_ms"] for s in stats])),
+                        "workgroups": stats[-1]["workgroups"], "lds_bitset": stats[-1]["lds_bitset"],
+                        "crit_ms": float(np.mean([s["crit_ms"] for s in stats])),
+                        "phase_ms": stats[-1]["phase_ms"]},
     }
     print(json.dumps(out), flush=True)
 

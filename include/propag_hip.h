@@ -58,12 +58,13 @@ typedef enum pg_status {
 
 /* Propagation schedule knobs.  Zero fields take the library defaults. */
 typedef struct pg_tuning {
-  uint32_t tile_w, tile_h;      /* cells per tile (default 16 x 16)            */
+  uint32_t tile_w, tile_h;      /* cells per tile (8 x 8; 0 = default)         */
   uint32_t workgroups;          /* persistent workgroups (0 = fill the device) */
   uint32_t max_iterations;      /* Jacobi sweep cap (0 = 64 * (W + H) + 1024)  */
   uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep */
   uint32_t spin_limit_log2;     /* device wait limit, 2^n polls (0 = 22)       */
-  uint32_t reserved[2];
+  uint32_t waves_per_group;     /* persistent kernel waves per workgroup (0 = auto) */
+  uint32_t profile;             /* 1: time the sweep phases (fills pg_stats.phase_ms) */
 } pg_tuning;
 
 /* Counters of one propagation (device-side counts copied back). */
@@ -75,6 +76,14 @@ typedef struct pg_stats {
   uint64_t tiles_visited;   /* tile visits (live tiles summed over sweeps)  */
   double   elapsed_ms;      /* device time of the whole call (HIP events)   */
   double   relax_ms;        /* device time of the relaxation kernel(s) only */
+  double   work_ms;         /* mean per workgroup: time in tile work          */
+  double   wait_ms;         /* mean per workgroup: time waiting at barriers   */
+  double   crit_ms;         /* sum over sweeps of the slowest workgroup's work */
+  uint32_t workgroups;      /* workgroups of the relaxation launch            */
+  uint32_t lds_bitset;      /* 1 if the DDA bitset was staged in LDS          */
+  double   phase_ms[4];     /* with tuning.profile: sum over sweeps of the
+                               slowest wave's time in halo staging, update
+                               rule, line-of-sight walks, commit + bookkeeping */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

@@ -42,13 +42,13 @@
 namespace pg {
 namespace {
 
-constexpr int kTile = 16;               // tile edge (cells)
+constexpr int kTile = 8;                // tile edge (cells): one cell per lane of a wave
 constexpr int kHalo = kTile + 2;        // staged edge
 constexpr int kHaloCells = kHalo * kHalo;
-constexpr int kThreads = kTile * kTile; // one thread per tile cell
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 constexpr int kStampNever = INT_MIN / 2;
 constexpr int kWalkBatch = 8;           // DDA cells tested per batch
+constexpr int kWave = 64;               // lanes per wavefront
 
 // ------------------------------------------------------------ control block
 
@@ -58,28 +58,19 @@ struct alignas(128) Line {
 
 // Grid-wide state of one propagation; each hot word sits on its own line.
 struct Control {
-  Line arrive;       // [0] barrier arrivals
-  Line gen;          // [0] barrier generation
+  Line gen;          // [0] barrier generation (sweep + 1 once released)
   Line live;         // [0] worklist length (grows during a sweep)
   Line book;         // [0] live snapshot, [1] done, [2] capped, [3] error, [4] last sweep
-  Line changes;      // [0..1] tiles changed in sweep k (k & 1)
-  unsigned long long stats[8];  // relaxations, improvements, dda_steps, tiles
+  Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
+  unsigned long long stats[8];  // relaxations, improvements, dda_steps, tiles, work, wait,
+                                // critical-path ticks
 };
 
-__device__ inline unsigned ld_acquire(const unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline unsigned ld_relaxed(const unsigned *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_relaxed(unsigned *p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_release(unsigned *p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline unsigned add_acq_rel(unsigned *p, unsigned v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline unsigned add_relaxed(unsigned *p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -149,84 +140,103 @@ __device__ inline float time_from(float from_t, int fx, int fy, int tx, int ty, 
   return from_t + (dist * (1.0f - proj / r)) * k.z;
 }
 
-// f32 DDA of propag.h:91-135.
-struct Dda {
-  int tx, ty, sx, sy, cx, cy;
-  float tmx, tmy, dlx, dly;
-  __device__ inline void init(int fx, int fy, int tx_, int ty_) {
-    tx = tx_;
-    ty = ty_;
-    sx = (0 < tx - fx) - (tx - fx < 0);
-    sy = (0 < ty - fy) - (ty - fy < 0);
-    cx = fx;
-    cy = fy;
-    tmx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(abs(tx - fx))));
-    tmy = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(abs(ty - fy))));
-    dlx = tmx;
-    dly = tmy;
-  }
-  __device__ inline bool next(int &rx, int &ry) {
-    rx = cx;
-    ry = cy;
-    if (cx == tx && cy == ty) {
-      cx += sx;
-      cy += sy;
-      return true;
-    }
-    const bool vx = sx > 0 ? cx <= tx : cx >= tx;
-    const bool vy = sy > 0 ? cy <= ty : cy >= ty;
-    if (!(vx && vy)) return false;
-    if (static_cast<double>(fabsf(tmx - tmy)) < 1e-9) {
-      cx += sx;
-      cy += sy;
-      tmx += dlx;
-      tmy += dly;
-    } else if (tmx < tmy) {
-      cx += sx;
-      tmx += dlx;
-    } else {
-      cy += sy;
-      tmy += dly;
-    }
-    return true;
-  }
-};
+// Reference words.  With both raster sides below 32768 (the usual case) a
+// reference is x | y << 16 with 15-bit coordinates and bit 15 caching
+// whether the DDA line from the cell to its reference is free of boundary
+// cells.  That walk (propag.cu:164-176) depends only on the pair (cell,
+// reference) and the static bitset, so it is computed once when a cell
+// commits a new reference instead of by every neighbour that evaluates it.
+// Larger rasters use full 16-bit coordinates and walk at evaluation time.
+constexpr uint32_t kClearBit = 0x8000u;
 
-__device__ inline bool bit_of(const uint32_t *words, uint32_t idx) {
-  return (words[idx >> 5] >> (idx & 31u)) & 1u;
+template <bool kCache>
+__device__ inline int ref_x(uint32_t r) { return static_cast<int>(r & (kCache ? 0x7fffu : 0xffffu)); }
+template <bool kCache>
+__device__ inline int ref_y(uint32_t r) { return static_cast<int>((r >> 16) & (kCache ? 0x7fffu : 0xffffu)); }
+__device__ inline uint32_t make_ref(int x, int y) {
+  return static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16);
 }
 
-// True if a boundary cell lies on the DDA from (fx,fy) to (tx,ty), both
-// included (propag.cu:164-176).  Cells are tested kWalkBatch at a time so the
-// bit loads of a batch are in flight together; `steps` counts the cells up to
-// and including the first boundary, as a serial walk would.
-__device__ inline bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *bits, int W,
-                                    uint32_t &steps) {
-  Dda d;
-  d.init(fx, fy, tx, ty);
+// Dynamic LDS of the persistent kernel: per-wave tile slices, then the
+// boundary bitset when it fits.
+extern __shared__ uint32_t dyn_lds[];
+
+template <bool kLdsBits>
+__device__ inline bool bit_at(const uint32_t *gbits, unsigned lds_off, uint32_t idx) {
+  const uint32_t w = kLdsBits ? dyn_lds[lds_off + (idx >> 5)] : gbits[idx >> 5];
+  return (w >> (idx & 31u)) & 1u;
+}
+
+// (double)d < 1e-9 for a float d >= 0 is exactly d <= this float (the float
+// nearest 1e-9 lies below it and the next one above).
+constexpr float kTie = 0x1.12e0bep-30f;
+
+// True if a boundary cell lies on the f32 DDA of propag.h:91-135 from
+// (fx,fy) to (tx,ty), both included; requires (fx,fy) != (tx,ty).  The walk
+// counts its steps u, v along each axis, so "cur is still between from and
+// to" (propag.h:117-120) is u <= |dx| && v <= |dy| and "cur == to" is
+// u == |dx| && v == |dy|; the linear index advances by precomputed
+// increments.  tmax/delta and the tie test are the reference's f32 values.
+// Cells are tested kWalkBatch at a time so a batch's bit loads are in flight
+// together; `steps` counts the cells up to and including the first boundary,
+// as a serial walk would.
+template <bool kLdsBits>
+__device__ bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *gbits, unsigned lds_off,
+                             int W, uint32_t &steps) {
+  const int ddx = tx - fx, ddy = ty - fy;
+  const int ax = abs(ddx), ay = abs(ddy);
+  const uint32_t ix = static_cast<uint32_t>((ddx > 0) - (ddx < 0));
+  const uint32_t iy = static_cast<uint32_t>(((ddy > 0) - (ddy < 0)) * W);
+  // 1.0 / float(abs(..)) is a double division narrowed to float
+  const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ax)));
+  const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ay)));
+  float tmx = dlx, tmy = dly;
+  int u = 0, v = 0;
+  uint32_t idx = static_cast<uint32_t>(fx) + static_cast<uint32_t>(fy) * static_cast<uint32_t>(W);
   bool alive = true;
   while (alive) {
-    uint32_t idx[kWalkBatch];
-    bool valid[kWalkBatch];
+    uint32_t id[kWalkBatch];
+    bool vis[kWalkBatch];
 #pragma unroll
     for (int b = 0; b < kWalkBatch; ++b) {
-      int x = 0, y = 0;
-      valid[b] = alive && d.next(x, y);
-      alive = valid[b];
-      idx[b] = valid[b] ? static_cast<uint32_t>(x) + static_cast<uint32_t>(y) * static_cast<uint32_t>(W) : 0u;
+      vis[b] = alive & (u <= ax) & (v <= ay);
+      id[b] = vis[b] ? idx : 0u;
+      alive = vis[b] & !((u == ax) & (v == ay));
+      const bool diag = fabsf(tmx - tmy) <= kTie;
+      const bool lt = tmx < tmy;
+      const bool xs = diag | lt, ys = diag | !lt;
+      u += xs ? 1 : 0;
+      v += ys ? 1 : 0;
+      idx += (xs ? ix : 0u) + (ys ? iy : 0u);
+      tmx += xs ? dlx : 0.0f;
+      tmy += ys ? dly : 0.0f;
     }
-    bool hit_bits[kWalkBatch];
+    uint32_t hits = 0;
 #pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b) hit_bits[b] = bit_of(bits, idx[b]);
-#pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b) {
-      if (valid[b]) {
-        ++steps;
-        if (hit_bits[b]) return true;
-      }
+    for (int b = 0; b < kWalkBatch; ++b) hits |= (bit_at<kLdsBits>(gbits, lds_off, id[b]) & vis[b]) ? (1u << b) : 0u;
+    if (hits) {
+      steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
+      return true;
     }
+#pragma unroll
+    for (int b = 0; b < kWalkBatch; ++b) steps += vis[b] ? 1u : 0u;
   }
   return false;
+}
+
+// Line-of-sight flag for a committed reference: the DDA from the cell to an
+// adjacent reference visits exactly those two cells.
+template <bool kLdsBits>
+__device__ inline bool line_clear(int x, int y, int rx, int ry, const uint32_t *gbits, unsigned lds_off,
+                                  int W, uint32_t &steps) {
+  if (rx == x && ry == y) return false;  // an ignition's own reference: never walked
+  if (abs(rx - x) <= 1 && abs(ry - y) <= 1) {
+    steps += 2;
+    const uint32_t a = static_cast<uint32_t>(x) + static_cast<uint32_t>(y) * static_cast<uint32_t>(W);
+    const uint32_t b = static_cast<uint32_t>(rx) + static_cast<uint32_t>(ry) * static_cast<uint32_t>(W);
+    return !bit_at<kLdsBits>(gbits, lds_off, a) && !bit_at<kLdsBits>(gbits, lds_off, b);
+  }
+  return !walk_blocked<kLdsBits>(x, y, rx, ry, gbits, lds_off, W, steps);
 }
 
 __device__ inline uint2 pack_state(float t, uint32_t ref) { return make_uint2(__float_as_uint(t), ref); }
@@ -234,32 +244,54 @@ __device__ inline uint2 pack_state(float t, uint32_t ref) { return make_uint2(__
 struct Args {
   Geom g;
   const float4 *coef;        // A, B, R, null flag
-  uint2 *st[2];              // double-buffered (time bits, refx | refy << 16)
+  uint2 *st[2];              // double-buffered (time bits, reference word)
   int *stamp;                // last sweep in which the cell changed
   const uint32_t *bits;      // boundary bitset (global copy)
   uint32_t *tile_live;       // 1 once a tile is on the worklist
   uint32_t *worklist;        // live tile ids
   Control *ctl;
+  struct Slot *slots;        // grid-barrier arrival lines, one per workgroup
   int ntx, nty;              // tiles per row / column
   unsigned max_iters;
   unsigned spin_limit;
   int lds_bits;              // bitset staged in LDS
+  int cache;                 // line-of-sight flags cached in reference words
+  unsigned long long *prof;  // [sweep][4] slowest-wave phase ticks, or null
+  unsigned prof_sweeps;      // rows of prof
 };
 
-struct TileLds {
+struct TileLds {  // one wave's staged tile (16x16 + 1-cell halo)
   float time[kHaloCells];
   uint32_t ref[kHaloCells];
   float rtime[kHaloCells];
   uint8_t chg[kHaloCells];
 };
+static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
+
+// Per-workgroup arrival slot of the grid barrier, one 128-byte line each:
+// arrive = (sweep + 1) << 32 | changed tiles, work = ticks spent in the sweep.
+struct alignas(128) Slot {
+  unsigned long long arrive;
+  unsigned long long work;
+  unsigned pad[28];
+};
 
 struct Counters {
   uint32_t relax, improve, steps, tiles;
+  uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
+  uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
 };
 
+__device__ inline void wave_sync() {
+  // Lanes of one wave exchange data through LDS: order the accesses.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Marks the 8 neighbour tiles of (tx,ty) live, appending newcomers.
-__device__ inline void grow_worklist(const Args &a, int tx, int ty) {
-  const int lane = threadIdx.x;
+__device__ inline void grow_worklist(const Args &a, int tx, int ty, int lane) {
   if (lane < 9 && lane != 4) {
     const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
     if (nx >= 0 && ny >= 0 && nx < a.ntx && ny < a.nty) {
@@ -276,49 +308,65 @@ __device__ inline void grow_worklist(const Args &a, int tx, int ty) {
   }
 }
 
-// One sweep over one tile.  Returns, uniformly over the workgroup, whether
-// any cell committed an improvement.
-__device__ int process_tile(const Args &a, const uint32_t *bits, int tile, int k, TileLds &s,
-                             Counters &cnt) {
+// One sweep over one tile by one wave: lane l owns cell (l % 8, l / 8) of
+// the 8x8 tile.  Returns, uniformly over the wave, whether any cell
+// committed an improvement.
+template <bool kCache, bool kLdsBits>
+__device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, TileLds &s, Counters &cnt) {
   const Geom &g = a.g;
   const uint2 *__restrict__ st_in = a.st[k & 1];
   uint2 *__restrict__ st_out = a.st[(k + 1) & 1];
   const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int ox = tx * kTile, oy = ty * kTile;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  uint64_t tp0 = a.prof ? wall_clock64() : 0;
 
-  // Stage the halo tile (load_points_into_shared_memory, propag.cu:231-261).
-  for (int h = tid; h < kHaloCells; h += kThreads) {
+  // Stage the halo tile (load_points_into_shared_memory, propag.cu:231-261)
+  // in two rounds of independent loads: the cells' state words and stamps,
+  // then their references' times and stamps (load_point, propag.cu:209-229).
+  constexpr int kPer = (kHaloCells + kWave - 1) / kWave;  // halo cells per lane
+  uint2 w[kPer];
+  int stp[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int h = lane + it * kWave;
     const int gx = ox + h % kHalo - 1, gy = oy + h / kHalo - 1;
-    float t = FLT_MAX, rt = FLT_MAX;
-    uint32_t ref = kNoRef;
-    bool chg = false;
-    if (gx >= 0 && gy >= 0 && gx < g.W && gy < g.H) {
-      const size_t gi = static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W;
-      const uint2 w = st_in[gi];
-      t = __uint_as_float(w.x);
-      ref = w.y;
-      chg = a.stamp[gi] >= k - 1;
-      // A neighbour only feeds the update rule while it burns before
-      // max_time; only then is its reference's live time needed.
-      if (t < g.max_time && t < FLT_MAX) {
-        const size_t ri = static_cast<size_t>(ref & 0xffffu) + static_cast<size_t>(ref >> 16) * g.W;
-        rt = __uint_as_float(st_in[ri].x);
-        chg = chg || a.stamp[ri] >= k - 1;
-      }
-    }
-    s.time[h] = t;
-    s.ref[h] = ref;
-    s.rtime[h] = rt;
-    s.chg[h] = chg ? 1 : 0;
+    const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
+    const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
+    w[it] = inb ? st_in[gi] : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
+    stp[it] = inb ? a.stamp[gi] : kStampNever;
   }
-  __syncthreads();
+  float rt[kPer];
+  int rstp[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const float t = __uint_as_float(w[it].x);
+    // A neighbour only feeds the update rule while it burns before
+    // max_time; only then is its reference's live time needed.
+    const bool gather = t < g.max_time && t < FLT_MAX;
+    const size_t ri =
+        gather ? static_cast<size_t>(ref_x<kCache>(w[it].y)) + static_cast<size_t>(ref_y<kCache>(w[it].y)) * g.W : 0;
+    rt[it] = gather ? __uint_as_float(st_in[ri].x) : FLT_MAX;
+    rstp[it] = gather ? a.stamp[ri] : kStampNever;
+  }
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int h = lane + it * kWave;
+    if (h < kHaloCells) {
+      s.time[h] = __uint_as_float(w[it].x);
+      s.ref[h] = w[it].y;
+      s.rtime[h] = rt[it];
+      s.chg[h] = (stp[it] >= k - 1 || rstp[it] >= k - 1) ? 1 : 0;
+    }
+  }
+  wave_sync();
+  uint64_t tp1 = a.prof ? wall_clock64() : 0;
+  uint64_t walk_ticks = 0;
 
-  const int lx = tid % kTile + 1, ly = tid / kTile + 1;
+  const int lx = (lane & (kTile - 1)) + 1, ly = lane / kTile + 1;
   const int gx = ox + lx - 1, gy = oy + ly - 1;
-  const bool inside = gx < g.W && gy < g.H;
   bool need = false;
-  if (inside) {
+  if (gx < g.W && gy < g.H) {
 #pragma unroll
     for (int j = -1; j < 2; ++j)
 #pragma unroll
@@ -327,6 +375,17 @@ __device__ int process_tile(const Args &a, const uint32_t *bits, int tile, int k
   bool improved = false;
   if (need) {
     ++cnt.relax;
+    float4 win[9];  // fire coefficients of the 3x3 neighbourhood
+#pragma unroll
+    for (int j = -1; j < 2; ++j) {
+#pragma unroll
+      for (int i = -1; i < 2; ++i) {
+        const int X = gx + i, Y = gy + j;
+        win[(j + 1) * 3 + (i + 1)] = (X >= 0 && Y >= 0 && X < g.W && Y < g.H)
+                                         ? a.coef[static_cast<size_t>(X) + static_cast<size_t>(Y) * g.W]
+                                         : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+      }
+    }
     const int me = lx + ly * kHalo;
     const float own_t = s.time[me];
     float best_t = own_t;
@@ -343,139 +402,234 @@ __device__ int process_tile(const Args &a, const uint32_t *bits, int tile, int k
         const int nl = (lx + i) + (ly + j) * kHalo;
         const float n_t = s.time[nl];
         if (!(n_t < g.max_time)) continue;
-        const float4 kf = a.coef[static_cast<size_t>(nx) + static_cast<size_t>(ny) * g.W];
+        const float4 kf = win[(j + 1) * 3 + (i + 1)];
         if (kf.w != 0.0f) continue;  // fire_is_null
         const uint32_t ref = s.ref[nl];
-        const int rfx = static_cast<int>(ref & 0xffffu), rfy = static_cast<int>(ref >> 16);
+        const int rfx = ref_x<kCache>(ref), rfy = ref_y<kCache>(ref);
         if (rfx == gx && rfy == gy) continue;
         const float nt = time_from(n_t, nx, ny, gx, gy, kf, g);
         const float ct = time_from(s.rtime[nl], rfx, rfy, gx, gy, kf, g);
         if (ct < nt && ct < best_t) {
-          uint32_t cand_ref = ref;
-          float cand_t = ct;
-          if (walk_blocked(nx, ny, rfx, rfy, bits, g.W, cnt.steps)) {
-            cand_ref = static_cast<uint32_t>(nx) | (static_cast<uint32_t>(ny) << 16);
-            cand_t = nt;
-          }
+          bool blocked;
+          if (kCache)
+            blocked = (ref & kClearBit) == 0u;
+          else
+            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, a.bits, lds_off, g.W, cnt.steps);
+          const uint32_t cand_ref = blocked ? make_ref(nx, ny) : make_ref(rfx, rfy);
+          const float cand_t = blocked ? nt : ct;
           if (cand_t < best_t) {
             best_t = cand_t;
             best_ref = cand_ref;
           }
         } else if (nt < best_t) {
           best_t = nt;
-          best_ref = static_cast<uint32_t>(nx) | (static_cast<uint32_t>(ny) << 16);
+          best_ref = make_ref(nx, ny);
         }
       }
     }
     const size_t gi = static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W;
     improved = best_t < own_t;  // update_point (propag.cu:192-200)
-    st_out[gi] = improved ? pack_state(best_t, best_ref) : pack_state(own_t, s.ref[me]);
     if (improved) {
+      const uint64_t tw = a.prof ? wall_clock64() : 0;
+      if (kCache &&
+          line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), a.bits, lds_off, g.W,
+                               cnt.steps))
+        best_ref |= kClearBit;
+      if (a.prof) walk_ticks += wall_clock64() - tw;
+      st_out[gi] = pack_state(best_t, best_ref);
       a.stamp[gi] = k;
       ++cnt.improve;
+    } else {
+      st_out[gi] = pack_state(own_t, s.ref[me]);
     }
   }
-  // Note: the barrier below also orders the LDS reuse by the next tile.
-  const int changed = __syncthreads_or(improved ? 1 : 0);
-  if (changed) grow_worklist(a, tx, ty);
-  if (tid == 0) ++cnt.tiles;
+  uint64_t tp2 = a.prof ? wall_clock64() : 0;
+  const bool changed = __ballot(improved) != 0ull;
+  if (changed) grow_worklist(a, tx, ty, lane);
+  if (lane == 0) ++cnt.tiles;
+  wave_sync();  // the next tile reuses this wave's LDS slice
+  if (a.prof) {
+    // per wave: staging, rule (minus walks), walks (slowest lane), tail
+    unsigned long long wk = walk_ticks;
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(wk, off);
+      wk = o > wk ? o : wk;
+    }
+    const uint64_t tp3 = wall_clock64();
+    cnt.ph[0] += tp1 - tp0;
+    cnt.ph[1] += (tp2 - tp1) > wk ? (tp2 - tp1) - wk : 0;
+    cnt.ph[2] += wk;
+    cnt.ph[3] += tp3 - tp2;
+  }
   return changed;
 }
 
-// Grid barrier with bookkeeping by the last arriver.  Returns 1 when the
-// propagation is finished (quiet sweep, iteration cap or timeout).
-__device__ int grid_sync(const Args &a, unsigned &gen, int k, unsigned my_changes) {
+__device__ inline unsigned long long ld_relaxed64(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier.  Every workgroup publishes its arrival with one release
+// store into its own line; wave 0 of workgroup 0 polls all lines with
+// relaxed loads, does the sweep's bookkeeping (quiet-sweep test, cap,
+// work-list snapshot, critical-path time) and releases the generation word,
+// which the other workgroups poll with relaxed loads.  Each side issues one
+// acquire fence after its wait.  Returns 1 when the propagation is finished.
+__device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t sweep_t0,
+                            Counters &cnt) {
   __shared__ int s_done;
   __syncthreads();
+  const unsigned long long target = static_cast<unsigned long long>(k) + 1ull;
+  Slot *slots = a.slots;
+  Control *c = a.ctl;
+  uint64_t t_arrive = 0;
   if (threadIdx.x == 0) {
-    Control *c = a.ctl;
-    if (my_changes) add_relaxed(&c->changes.v[k & 1], my_changes);
-    __threadfence();
-    const unsigned prev = add_acq_rel(&c->arrive.v[0], 1u);
-    int done = 0;
-    if (prev == gridDim.x - 1) {
-      const unsigned ch = ld_relaxed(&c->changes.v[k & 1]);
-      st_relaxed(&c->changes.v[(k + 1) & 1], 0u);
-      st_relaxed(&c->book.v[0], ld_relaxed(&c->live.v[0]));
-      const bool cap = static_cast<unsigned>(k + 1) >= a.max_iters;
-      done = (ch == 0u) || cap;
-      if (cap && ch != 0u) st_relaxed(&c->book.v[2], 1u);
-      if (done) st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
-      st_relaxed(&c->book.v[1], done ? 1u : 0u);
-      st_relaxed(&c->arrive.v[0], 0u);
-      st_release(&c->gen.v[0], gen + 1u);
-    } else {
-      unsigned spins = 0;
-      while (ld_acquire(&c->gen.v[0]) == gen) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > a.spin_limit) {
-          st_relaxed(&c->book.v[3], 1u);
-          break;
+    t_arrive = wall_clock64();
+    cnt.work_ticks += t_arrive - sweep_t0;
+    __hip_atomic_store(&slots[blockIdx.x].work, static_cast<unsigned long long>(t_arrive - sweep_t0),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&slots[blockIdx.x].arrive, (target << 32) | wg_changes, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < kWave) {
+      const int lane = threadIdx.x;
+      unsigned long long changes = 0, maxwork = 0;
+      bool timeout = false;
+      for (unsigned i = lane; i < gridDim.x; i += kWave) {
+        unsigned spins = 0;
+        unsigned long long v;
+        while (((v = ld_relaxed64(&slots[i].arrive)) >> 32) != target) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > a.spin_limit) {
+            timeout = true;
+            break;
+          }
         }
+        changes += v & 0xffffffffull;
       }
-      done = ld_relaxed(&c->book.v[1]) != 0u || ld_relaxed(&c->book.v[3]) != 0u;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      for (unsigned i = lane; i < gridDim.x; i += kWave) {
+        const unsigned long long w = ld_relaxed64(&slots[i].work);
+        maxwork = w > maxwork ? w : maxwork;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        changes += __shfl_xor(changes, off);
+        const unsigned long long o = __shfl_xor(maxwork, off);
+        maxwork = o > maxwork ? o : maxwork;
+      }
+      timeout = __ballot(timeout) != 0ull;
+      if (lane == 0) {
+        const bool cap = static_cast<unsigned>(k + 1) >= a.max_iters;
+        const bool done = changes == 0ull || cap || timeout;
+        st_relaxed(&c->book.v[0], ld_relaxed(&c->live.v[0]));
+        if (cap && changes != 0ull) st_relaxed(&c->book.v[2], 1u);
+        if (timeout) st_relaxed(&c->book.v[3], 1u);
+        if (done) st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
+        st_relaxed(&c->book.v[1], done ? 1u : 0u);
+        c->stats[6] += maxwork;  // only this lane ever writes it
+        __hip_atomic_store(&c->gen.v[0], static_cast<unsigned>(target), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        s_done = done ? 1 : 0;
+      }
     }
-    __threadfence();
-    gen += 1u;
-    s_done = done;
+  } else if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    bool timeout = false;
+    while (ld_relaxed(&c->gen.v[0]) != static_cast<unsigned>(target)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > a.spin_limit) {
+        timeout = true;
+        st_relaxed(&c->book.v[3], 1u);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_done = (timeout || ld_relaxed(&c->book.v[1]) != 0u || ld_relaxed(&c->book.v[3]) != 0u) ? 1 : 0;
   }
   __syncthreads();
+  if (threadIdx.x == 0) cnt.wait_ticks += wall_clock64() - t_arrive;
   return s_done;
 }
 
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
-  unsigned long long v[4] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles};
+  unsigned long long v[6] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
+                             cnt.wait_ticks};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 6; ++q) {
     unsigned long long x = v[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(&a.ctl->stats[q], x);
   }
 }
 
-// Persistent form: one launch runs every sweep; workgroups stride over the
-// live worklist and meet at the grid barrier after each sweep.
-__global__ __launch_bounds__(kThreads) void propagate_persistent(Args a) {
-  extern __shared__ uint32_t dyn_lds[];
-  __shared__ TileLds s;
-  const uint32_t *bits = a.bits;
-  if (a.lds_bits) {
+// Persistent form: one launch runs every sweep.  Each wave strides over the
+// live work list, one tile at a time; workgroups meet at the grid barrier
+// after each sweep.  Dynamic LDS: one TileLds per wave, then (optionally)
+// the boundary bitset.
+template <bool kCache, bool kLdsBits>
+__global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
+  __shared__ unsigned s_changes;
+  const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
+  TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds);
+  TileLds &s = slices[wave];
+  const unsigned lds_off = nw * static_cast<unsigned>(sizeof(TileLds) / 4);
+  if (threadIdx.x == 0) s_changes = 0;
+  if (kLdsBits) {
     const unsigned nwords = (static_cast<unsigned>(a.g.W) * static_cast<unsigned>(a.g.H) + 31u) / 32u;
-    for (unsigned i = threadIdx.x; i < nwords; i += blockDim.x) dyn_lds[i] = a.bits[i];
-    __syncthreads();
-    bits = dyn_lds;
+    for (unsigned i = threadIdx.x; i < nwords; i += blockDim.x) dyn_lds[lds_off + i] = a.bits[i];
   }
-  Counters cnt{0, 0, 0, 0};
-  unsigned gen = 0;
+  __syncthreads();
+  Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
+  uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
+  const unsigned gwave = blockIdx.x * nw + wave, nwaves = gridDim.x * nw;
   for (int k = 0;; ++k) {
     const unsigned n_live = ld_relaxed(&a.ctl->book.v[0]);
     unsigned my_changes = 0;
-    for (unsigned w = blockIdx.x; w < n_live; w += gridDim.x) {
-      const int tile = static_cast<int>(a.worklist[w]);
-      my_changes += process_tile(a, bits, tile, k, s, cnt) ? 1u : 0u;
-      __syncthreads();
+    for (unsigned w = gwave; w < n_live; w += nwaves)
+      my_changes +=
+          wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(a.worklist[w]), k, s, cnt) ? 1u : 0u;
+    if ((threadIdx.x & (kWave - 1)) == 0 && my_changes) atomicAdd(&s_changes, my_changes);
+    if (a.prof && (threadIdx.x & (kWave - 1)) == 0 && static_cast<unsigned>(k) < a.prof_sweeps) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (cnt.ph[q]) atomicMax(&a.prof[static_cast<size_t>(k) * 4 + q], static_cast<unsigned long long>(cnt.ph[q]));
+        cnt.ph[q] = 0;
+      }
     }
-    if (grid_sync(a, gen, k, my_changes)) break;
+    __syncthreads();
+    const unsigned wg_changes = s_changes;
+    const int done = grid_barrier(a, k, wg_changes, t0, cnt);
+    if (threadIdx.x == 0) {
+      s_changes = 0;
+      t0 = wall_clock64();
+    }
+    if (done) break;
   }
   flush_counters(a, cnt);
 }
 
 // Plain form: one launch per sweep (the reference's relaunch structure).
-__global__ __launch_bounds__(kThreads) void propagate_sweep(Args a, int k) {
-  __shared__ TileLds s;
-  Counters cnt{0, 0, 0, 0};
+template <bool kCache>
+__global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
+  __shared__ TileLds slices[256 / kWave];
+  __shared__ unsigned s_changes;
+  const int wave = threadIdx.x / kWave;
+  if (threadIdx.x == 0) s_changes = 0;
+  __syncthreads();
+  Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   const unsigned n_live = ld_relaxed(&a.ctl->book.v[0]);
+  const unsigned nw = blockDim.x / kWave;
   unsigned my_changes = 0;
-  for (unsigned w = blockIdx.x; w < n_live; w += gridDim.x) {
-    my_changes += process_tile(a, a.bits, static_cast<int>(a.worklist[w]), k, s, cnt) ? 1u : 0u;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && my_changes) add_relaxed(&a.ctl->changes.v[k & 1], my_changes);
+  for (unsigned w = blockIdx.x * nw + wave; w < n_live; w += gridDim.x * nw)
+    my_changes += wave_tile<kCache, false>(a, 0, static_cast<int>(a.worklist[w]), k, slices[wave], cnt) ? 1u : 0u;
+  if ((threadIdx.x & (kWave - 1)) == 0 && my_changes) atomicAdd(&s_changes, my_changes);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_changes) add_relaxed(&a.ctl->changes.v[k & 1], s_changes);
   flush_counters(a, cnt);
 }
 
-// Between plain sweeps: the last-arriver bookkeeping of grid_sync.
+// Between plain sweeps: the bookkeeping the persistent barrier does.
 __global__ void sweep_book(Args a, int k) {
   Control *c = a.ctl;
   const unsigned ch = c->changes.v[k & 1];
@@ -497,8 +651,9 @@ __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
 }
 
 // Fire coefficients, packed double-buffered state, stamps, reference
-// validation and the initial worklist (tiles holding or touching a burned
-// cell).
+// validation, the line-of-sight flags of the initial references (needs the
+// boundary bitset, so it runs after boundaries_kernel) and the initial
+// worklist (tiles holding or touching a burned cell).
 __global__ void prep(Args a, const float *speed, const float *az, const float *ecc,
                      const float *time, const uint16_t *rx, const uint16_t *ry) {
   const size_t n = static_cast<size_t>(a.g.W) * a.g.H;
@@ -514,17 +669,24 @@ __global__ void prep(Args a, const float *speed, const float *az, const float *e
     k.w = s == 0.0f ? 1.0f : 0.0f;
     const_cast<float4 *>(a.coef)[i] = k;
     const float t = time[i];
-    const uint32_t ref = static_cast<uint32_t>(rx[i]) | (static_cast<uint32_t>(ry[i]) << 16);
+    uint32_t ref = static_cast<uint32_t>(rx[i]) | (static_cast<uint32_t>(ry[i]) << 16);
+    const bool burned = t < FLT_MAX;
+    const int x = static_cast<int>(i % a.g.W), y = static_cast<int>(i / a.g.W);
+    if (burned && (rx[i] >= a.g.W || ry[i] >= a.g.H)) {
+      atomicOr(&a.ctl->book.v[3], 2u);  // invalid reference
+      a.st[0][i] = pack_state(FLT_MAX, kNoRef);
+      a.st[1][i] = pack_state(FLT_MAX, kNoRef);
+      a.stamp[i] = kStampNever;
+      continue;
+    }
+    if (burned && a.cache) {
+      uint32_t steps = 0;
+      if (line_clear<false>(x, y, rx[i], ry[i], a.bits, 0, a.g.W, steps)) ref |= kClearBit;
+    }
     a.st[0][i] = pack_state(t, ref);
     a.st[1][i] = pack_state(t, ref);
-    const bool burned = t < FLT_MAX;
     a.stamp[i] = burned ? -1 : kStampNever;
     if (burned) {
-      if (rx[i] >= a.g.W || ry[i] >= a.g.H) {
-        atomicOr(&a.ctl->book.v[3], 2u);  // invalid reference
-        continue;
-      }
-      const int x = static_cast<int>(i % a.g.W), y = static_cast<int>(i / a.g.W);
       const int tx = x / kTile, ty = y / kTile;
       for (int q = 0; q < 9; ++q) {
         const int nx = tx + q % 3 - 1, ny = ty + q / 3 - 1;
@@ -584,10 +746,13 @@ __global__ void post(Args a, int final_buf, const float *speed, float *time, uin
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const uint2 w = st[i];
     float t = __uint_as_float(w.x);
+    // A cell never reached keeps the caller's references untouched.
+    if (t < FLT_MAX) {
+      rx[i] = static_cast<uint16_t>(a.cache ? ref_x<true>(w.y) : ref_x<false>(w.y));
+      ry[i] = static_cast<uint16_t>(a.cache ? ref_y<true>(w.y) : ref_y<false>(w.y));
+    }
     if (speed[i] == 0.0f || t > a.g.max_time) t = FLT_MAX;
     time[i] = t;
-    rx[i] = static_cast<uint16_t>(w.y & 0xffffu);
-    ry[i] = static_cast<uint16_t>(w.y >> 16);
   }
 }
 
@@ -640,8 +805,8 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     return set_error(PG_ERR_INVALID, "max_time must be finite");
   pg_tuning tn{};
   if (tuning) tn = *tuning;
-  if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 16");
-  if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 16");
+  if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 8");
+  if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 8");
   const float gdx = settings->geo_ref.transform.gt.dx, gdy = settings->geo_ref.transform.gt.dy;
 
   Args a{};
@@ -653,6 +818,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   a.g.max_time = settings->max_time;
   a.g.north_up = gdy < 0.0f;
   a.g.square = fabsf(gdx) == fabsf(gdy);
+  a.cache = (W <= 0x7fffu && H <= 0x7fffu) ? 1 : 0;
   a.ntx = static_cast<int>((W + kTile - 1) / kTile);
   a.nty = static_cast<int>((H + kTile - 1) / kTile);
   a.max_iters = tn.max_iterations ? tn.max_iterations : 64u * (W + H) + 1024u;
@@ -670,6 +836,10 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   const size_t o_live = off; off = align_up(off + ntiles * sizeof(uint32_t), 256);
   const size_t o_work = off; off = align_up(off + ntiles * sizeof(uint32_t), 256);
   const size_t o_ctl = off; off = align_up(off + sizeof(Control), 256);
+  const size_t max_groups = static_cast<size_t>(dev->cu_count) * 32;
+  const size_t o_slots = off; off = align_up(off + max_groups * sizeof(Slot), 256);
+  const unsigned prof_sweeps = tn.profile ? 8192u : 0u;
+  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(prof_sweeps) * 4 * sizeof(unsigned long long), 256);
   void *ws = nullptr;
   pg_status st = workspace(dev->device, off, &ws);
   if (st != PG_OK) return st;
@@ -681,6 +851,10 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   a.tile_live = reinterpret_cast<uint32_t *>(base + o_live);
   a.worklist = reinterpret_cast<uint32_t *>(base + o_work);
   a.ctl = reinterpret_cast<Control *>(base + o_ctl);
+  a.slots = reinterpret_cast<Slot *>(base + o_slots);
+  a.prof = prof_sweeps ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
+  a.prof_sweeps = prof_sweeps;
+  if (prof_sweeps) PG_HIP(hipMemsetAsync(a.prof, 0, static_cast<size_t>(prof_sweeps) * 4 * 8, stream));
   a.bits = boundaries;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evr0 = nullptr, evr1 = nullptr;
@@ -706,9 +880,11 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   hipLaunchKernelGGL(fill_u32, grid_for(ntiles), dim3(256), 0, stream, a.tile_live, ntiles, 0u);
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
                      sizeof(Control) / 4, 0u);
-  hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, a, speed, az, ecc, time, refs_x, refs_y);
+  hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t *>(a.slots), max_groups * sizeof(Slot) / 4, 0u);
   hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, a.g, speed, az, ecc,
                      boundaries);
+  hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, a, speed, az, ecc, time, refs_x, refs_y);
   hipLaunchKernelGGL(init_control, dim3(1), dim3(1), 0, stream, a.ctl);
   PG_HIP(hipGetLastError());
 
@@ -721,10 +897,15 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   }
 
   PG_HIP(hipEventRecord(evr0, stream));
+  unsigned launched_groups = 0;
   if (tn.mode == 1) {
     const dim3 grid(cu * 4);
+    launched_groups = grid.x;
     for (int k = 0;; ++k) {
-      hipLaunchKernelGGL(propagate_sweep, grid, dim3(kThreads), 0, stream, a, k);
+      if (a.cache)
+        hipLaunchKernelGGL(propagate_sweep<true>, grid, dim3(256), 0, stream, a, k);
+      else
+        hipLaunchKernelGGL(propagate_sweep<false>, grid, dim3(256), 0, stream, a, k);
       hipLaunchKernelGGL(sweep_book, dim3(1), dim3(1), 0, stream, a, k);
       PG_HIP(hipGetLastError());
       PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
@@ -733,17 +914,33 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     }
     PG_HIP(hipEventRecord(evr1, stream));
   } else {
+    // Waves per workgroup: as many tile slices as fit beside the bitset when
+    // the bitset can live in LDS (one workgroup per CU), else 16 waves.
     const size_t bits_bytes = nwords * sizeof(uint32_t);
-    a.lds_bits = bits_bytes + sizeof(TileLds) + 1024 <= dev->lds_per_block ? 1 : 0;
-    const size_t shmem = a.lds_bits ? bits_bytes : 0;
+    const size_t static_lds = 256;
+    int waves = 16;
+    a.lds_bits = 0;
+    if (bits_bytes + static_lds + sizeof(TileLds) * 4 <= dev->lds_per_block) {
+      a.lds_bits = 1;
+      waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / sizeof(TileLds));
+    }
+    const int max_waves = 16;  // the kernel's launch bound: 1024 threads
+    if (tn.waves_per_group) waves = static_cast<int>(tn.waves_per_group);
+    if (waves > max_waves) waves = max_waves;
+    const size_t shmem = waves * sizeof(TileLds) + (a.lds_bits ? bits_bytes : 0);
+    const int block = waves * kWave;
+    void (*kern)(Args) = a.cache ? (a.lds_bits ? propagate_persistent<true, true> : propagate_persistent<true, false>)
+                                 : (a.lds_bits ? propagate_persistent<false, true> : propagate_persistent<false, false>);
     int per_cu = 0;
-    PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, propagate_persistent, kThreads, shmem));
+    PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, shmem));
     if (per_cu < 1) return set_error(PG_ERR_INVALID, "persistent kernel does not fit on a CU");
     unsigned groups = cu * static_cast<unsigned>(per_cu);
+    if (groups > max_groups) groups = static_cast<unsigned>(max_groups);
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
+    launched_groups = groups;
     void *kargs[] = {&a};
-    PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(propagate_persistent), dim3(groups),
-                                      dim3(kThreads), kargs, static_cast<unsigned>(shmem), stream));
+    PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups),
+                                      dim3(block), kargs, static_cast<unsigned>(shmem), stream));
     PG_HIP(hipEventRecord(evr1, stream));
     PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
@@ -769,6 +966,22 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     stats->tiles_visited = counters[3];
     stats->elapsed_ms = ms;
     stats->relax_ms = rms;
+    int clock_khz = 0;
+    if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev->device) != hipSuccess ||
+        clock_khz <= 0)
+      clock_khz = 100000;  // 100 MHz constant clock
+    const double groups_d = launched_groups ? static_cast<double>(launched_groups) : 1.0;
+    stats->work_ms = static_cast<double>(counters[4]) / clock_khz / groups_d;
+    stats->wait_ms = static_cast<double>(counters[5]) / clock_khz / groups_d;
+    stats->crit_ms = static_cast<double>(counters[6]) / clock_khz;
+    stats->workgroups = launched_groups;
+    stats->lds_bitset = static_cast<uint32_t>(a.lds_bits);
+    for (int q = 0; q < 4; ++q) stats->phase_ms[q] = 0.0;
+    if (prof_sweeps) {
+      std::vector<unsigned long long> prof(static_cast<size_t>(prof_sweeps) * 4);
+      PG_HIP(hipMemcpy(prof.data(), a.prof, prof.size() * 8, hipMemcpyDeviceToHost));
+      for (size_t r = 0; r < prof.size(); ++r) stats->phase_ms[r % 4] += static_cast<double>(prof[r]) / clock_khz;
+    }
   }
   if (book[2]) return set_error(PG_ERR_NOT_CONVERGED, "iteration cap reached before convergence");
   return PG_OK;

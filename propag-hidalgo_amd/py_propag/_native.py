@@ -38,17 +38,25 @@ class SettingsC(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("workgroups", C.c_uint32),
                 ("max_iterations", C.c_uint32), ("mode", C.c_uint32),
-                ("spin_limit_log2", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("spin_limit_log2", C.c_uint32), ("waves_per_group", C.c_uint32),
+                ("profile", C.c_uint32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_uint64), ("relaxations", C.c_uint64),
                 ("improvements", C.c_uint64), ("dda_steps", C.c_uint64),
                 ("tiles_visited", C.c_uint64), ("elapsed_ms", C.c_double),
-                ("relax_ms", C.c_double)]
+                ("relax_ms", C.c_double), ("work_ms", C.c_double), ("wait_ms", C.c_double),
+                ("crit_ms", C.c_double),
+                ("workgroups", C.c_uint32), ("lds_bitset", C.c_uint32),
+                ("phase_ms", C.c_double * 4)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "phase_ms"}
+        d["phase_ms"] = list(self.phase_ms)
+        return d
+
+
 
 
 STATUS = {0: "PG_OK", 1: "PG_ERR_INVALID", 2: "PG_ERR_HIP", 3: "PG_ERR_NO_DEVICE",
