@@ -126,7 +126,8 @@ def main():
         f"{stats[-1]['iterations']} sweeps, {relax} relaxations, {burned} burned; per workgroup "
         f"work {stats[-1]['work_ms']:.3f} ms, barrier wait {stats[-1]['wait_ms']:.3f} ms, "
         f"{stats[-1]['workgroups']} workgroups, LDS bitset {stats[-1]['lds_bitset']}, critical path "
-        f"{stats[-1]['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) {stats[-1]['phase_ms']}")
+        f"{stats[-1]['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) {stats[-1]['phase_ms']}, "
+        f"barrier (start/span/gather/book) {stats[-1]['barrier_ms']}")
 
     # ---------------------------------------------------- Rothermel field
     field = synth.rothermel_field(seed=0, side=1024)
@@ -211,7 +212,8 @@ def main():
                         "wait_ms": float(np.mean([s["wait_ms"] for s in stats])),
                         "workgroups": stats[-1]["workgroups"], "lds_bitset": stats[-1]["lds_bitset"],
                         "crit_ms": float(np.mean([s["crit_ms"] for s in stats])),
-                        "phase_ms": stats[-1]["phase_ms"]},
+                        "phase_ms": stats[-1]["phase_ms"],
+                        "barrier_ms": stats[-1]["barrier_ms"]},
     }
     print(json.dumps(out), flush=True)
 

@@ -84,6 +84,11 @@ typedef struct pg_stats {
   double   phase_ms[4];     /* with tuning.profile: sum over sweeps of the
                                slowest wave's time in halo staging, update
                                rule, line-of-sight walks, commit + bookkeeping */
+  double   barrier_ms[4];   /* with tuning.profile: sum over sweeps of the time
+                               from a barrier release until the last workgroup
+                               started, the longest workgroup span, last
+                               arrival until the master had seen all arrivals,
+                               and the master's bookkeeping                   */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

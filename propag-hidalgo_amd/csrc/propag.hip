@@ -172,56 +172,60 @@ __device__ inline bool bit_at(const uint32_t *gbits, unsigned lds_off, uint32_t 
 constexpr float kTie = 0x1.12e0bep-30f;
 
 // True if a boundary cell lies on the f32 DDA of propag.h:91-135 from
-// (fx,fy) to (tx,ty), both included; requires (fx,fy) != (tx,ty).  The walk
-// counts its steps u, v along each axis, so "cur is still between from and
-// to" (propag.h:117-120) is u <= |dx| && v <= |dy| and "cur == to" is
-// u == |dx| && v == |dy|; the linear index advances by precomputed
-// increments.  tmax/delta and the tie test are the reference's f32 values.
-// Cells are tested kWalkBatch at a time so a batch's bit loads are in flight
-// together; `steps` counts the cells up to and including the first boundary,
-// as a serial walk would.
+// (fx,fy) to (tx,ty), both included.  ru, rv count the steps left along each
+// axis: the reference's "cur is still between from and to" (propag.h:117-120)
+// is ru >= 0 && rv >= 0, i.e. (ru | rv) >= 0, and once the walk has visited
+// `to` or stepped past it one of them is negative for good.  With
+// d = tmax_x - tmax_y (exact whenever |d| is near the tie threshold, the two
+// being within a factor of two of each other), the reference's branches
+// become: x advances iff d <= kTie (tie or tmax_x < tmax_y), y advances iff
+// not d < -kTie (tie or tmax_x >= tmax_y; also for from == to, where d is
+// NaN, so that walk visits its one cell and ends).  Cells are tested
+// kWalkBatch at a time so a batch's bit loads are in flight together;
+// `steps` counts the cells up to and including the first boundary, as a
+// serial walk would.
 template <bool kLdsBits>
 __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *gbits, unsigned lds_off,
                              int W, uint32_t &steps) {
   const int ddx = tx - fx, ddy = ty - fy;
-  const int ax = abs(ddx), ay = abs(ddy);
-  const uint32_t ix = static_cast<uint32_t>((ddx > 0) - (ddx < 0));
-  const uint32_t iy = static_cast<uint32_t>(((ddy > 0) - (ddy < 0)) * W);
+  int ru = abs(ddx), rv = abs(ddy);
+  const int ix = (ddx > 0) - (ddx < 0);
+  const int iy = ((ddy > 0) - (ddy < 0)) * W;
   // 1.0 / float(abs(..)) is a double division narrowed to float
-  const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ax)));
-  const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ay)));
+  const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ru)));
+  const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(rv)));
   float tmx = dlx, tmy = dly;
-  int u = 0, v = 0;
-  uint32_t idx = static_cast<uint32_t>(fx) + static_cast<uint32_t>(fy) * static_cast<uint32_t>(W);
-  bool alive = true;
-  while (alive) {
+  int idx = fx + fy * W;
+  for (;;) {
     uint32_t id[kWalkBatch];
-    bool vis[kWalkBatch];
+    int live[kWalkBatch];
 #pragma unroll
     for (int b = 0; b < kWalkBatch; ++b) {
-      vis[b] = alive & (u <= ax) & (v <= ay);
-      id[b] = vis[b] ? idx : 0u;
-      alive = vis[b] & !((u == ax) & (v == ay));
-      const bool diag = fabsf(tmx - tmy) <= kTie;
-      const bool lt = tmx < tmy;
-      const bool xs = diag | lt, ys = diag | !lt;
-      u += xs ? 1 : 0;
-      v += ys ? 1 : 0;
-      idx += (xs ? ix : 0u) + (ys ? iy : 0u);
-      tmx += xs ? dlx : 0.0f;
-      tmy += ys ? dly : 0.0f;
+      live[b] = ru | rv;
+      id[b] = live[b] >= 0 ? static_cast<uint32_t>(idx) : 0u;
+      const float d = tmx - tmy;
+      const bool xs = d <= kTie, ys = !(d < -kTie);
+      ru -= xs ? 1 : 0;
+      rv -= ys ? 1 : 0;
+      idx += (xs ? ix : 0) + (ys ? iy : 0);
+      tmx = xs ? tmx + dlx : tmx;
+      tmy = ys ? tmy + dly : tmy;
     }
     uint32_t hits = 0;
 #pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b) hits |= (bit_at<kLdsBits>(gbits, lds_off, id[b]) & vis[b]) ? (1u << b) : 0u;
+    for (int b = 0; b < kWalkBatch; ++b)
+      hits |= (bit_at<kLdsBits>(gbits, lds_off, id[b]) && live[b] >= 0) ? (1u << b) : 0u;
     if (hits) {
       steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
       return true;
     }
+    if ((ru | rv) < 0) {
 #pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b) steps += vis[b] ? 1u : 0u;
+      for (int b = 0; b < kWalkBatch; ++b) steps += live[b] >= 0 ? 1u : 0u;
+      return false;
+    }
+    steps += kWalkBatch;
   }
-  return false;
 }
 
 // Line-of-sight flag for a committed reference: the DDA from the cell to an
@@ -256,7 +260,7 @@ struct Args {
   unsigned spin_limit;
   int lds_bits;              // bitset staged in LDS
   int cache;                 // line-of-sight flags cached in reference words
-  unsigned long long *prof;  // [sweep][4] slowest-wave phase ticks, or null
+  unsigned long long *prof;  // [sweep][kProfCols] profile ticks, or null
   unsigned prof_sweeps;      // rows of prof
 };
 
@@ -273,8 +277,17 @@ static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
 struct alignas(128) Slot {
   unsigned long long arrive;
   unsigned long long work;
-  unsigned pad[28];
+  unsigned long long ph[4];  // profiling: slowest wave's phase ticks
+  unsigned long long t_seen, t_arrive;  // profiling: sweep start / arrival
+  unsigned pad[16];
 };
+static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
+
+// Profile columns per sweep: slowest wave per phase (staging, update rule,
+// line-of-sight walks, commit and work-list), then the barrier: release of
+// the previous sweep -> last workgroup starting, the longest workgroup span,
+// last arrival -> master has seen all, master bookkeeping.
+constexpr int kProfCols = 8;
 
 struct Counters {
   uint32_t relax, improve, steps, tiles;
@@ -475,7 +488,8 @@ __device__ inline unsigned long long ld_relaxed64(const unsigned long long *p) {
 // which the other workgroups poll with relaxed loads.  Each side issues one
 // acquire fence after its wait.  Returns 1 when the propagation is finished.
 __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t sweep_t0,
-                            Counters &cnt) {
+                            Counters &cnt, const unsigned long long *wg_ph) {
+  __shared__ unsigned long long s_release;  // master: time of the last release
   __shared__ int s_done;
   __syncthreads();
   const unsigned long long target = static_cast<unsigned long long>(k) + 1ull;
@@ -487,6 +501,11 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
     cnt.work_ticks += t_arrive - sweep_t0;
     __hip_atomic_store(&slots[blockIdx.x].work, static_cast<unsigned long long>(t_arrive - sweep_t0),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a.prof) {
+      for (int q = 0; q < 4; ++q) slots[blockIdx.x].ph[q] = wg_ph[q];
+      slots[blockIdx.x].t_seen = sweep_t0;
+      slots[blockIdx.x].t_arrive = t_arrive;
+    }
     __hip_atomic_store(&slots[blockIdx.x].arrive, (target << 32) | wg_changes, __ATOMIC_RELEASE,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -517,6 +536,27 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
         const unsigned long long o = __shfl_xor(maxwork, off);
         maxwork = o > maxwork ? o : maxwork;
       }
+      if (a.prof && static_cast<unsigned>(k) < a.prof_sweeps) {
+        const unsigned long long t_all = wall_clock64();
+        // k == 0: the kernel start of workgroup 0 stands in for a release
+        const unsigned long long rel = k == 0 ? __shfl(sweep_t0, 0) : s_release;
+        unsigned long long m[7] = {0, 0, 0, 0, 0, 0, 0};  // ph[4], start, span, arrival
+        for (unsigned i = lane; i < gridDim.x; i += kWave) {
+          for (int q = 0; q < 4; ++q) m[q] = max(m[q], slots[i].ph[q]);
+          const unsigned long long ts = slots[i].t_seen, ta = slots[i].t_arrive;
+          m[4] = max(m[4], ts > rel ? ts - rel : 0ull);
+          m[5] = max(m[5], ta - ts);
+          m[6] = max(m[6], ta);
+        }
+        for (int q = 0; q < 7; ++q)
+          for (int off = 32; off > 0; off >>= 1) m[q] = max(m[q], __shfl_xor(m[q], off));
+        if (lane == 0) {
+          unsigned long long *row = a.prof + static_cast<size_t>(k) * kProfCols;
+          for (int q = 0; q < 6; ++q) row[q] = m[q];
+          row[6] = t_all > m[6] ? t_all - m[6] : 0ull;
+          row[7] = t_all;  // completed below with the release time
+        }
+      }
       timeout = __ballot(timeout) != 0ull;
       if (lane == 0) {
         const bool cap = static_cast<unsigned>(k + 1) >= a.max_iters;
@@ -527,6 +567,13 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
         if (done) st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
         st_relaxed(&c->book.v[1], done ? 1u : 0u);
         c->stats[6] += maxwork;  // only this lane ever writes it
+        if (a.prof) {
+          s_release = wall_clock64();
+          if (static_cast<unsigned>(k) < a.prof_sweeps) {
+            unsigned long long *row = a.prof + static_cast<size_t>(k) * kProfCols;
+            row[7] = s_release - row[7];
+          }
+        }
         __hip_atomic_store(&c->gen.v[0], static_cast<unsigned>(target), __ATOMIC_RELEASE,
                            __HIP_MEMORY_SCOPE_AGENT);
         s_done = done ? 1 : 0;
@@ -570,11 +617,13 @@ __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
 template <bool kCache, bool kLdsBits>
 __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   __shared__ unsigned s_changes;
+  __shared__ unsigned long long s_ph[4];
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds);
   TileLds &s = slices[wave];
   const unsigned lds_off = nw * static_cast<unsigned>(sizeof(TileLds) / 4);
   if (threadIdx.x == 0) s_changes = 0;
+  if (threadIdx.x < 4) s_ph[threadIdx.x] = 0;
   if (kLdsBits) {
     const unsigned nwords = (static_cast<unsigned>(a.g.W) * static_cast<unsigned>(a.g.H) + 31u) / 32u;
     for (unsigned i = threadIdx.x; i < nwords; i += blockDim.x) dyn_lds[lds_off + i] = a.bits[i];
@@ -582,7 +631,10 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   __syncthreads();
   Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
-  const unsigned gwave = blockIdx.x * nw + wave, nwaves = gridDim.x * nw;
+  // Work-list entry w goes to wave w / gridDim.x of workgroup w % gridDim.x:
+  // tiles found together (neighbours on the fire front) land on different
+  // CUs instead of queueing on one CU's SIMDs.
+  const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   for (int k = 0;; ++k) {
     const unsigned n_live = ld_relaxed(&a.ctl->book.v[0]);
     unsigned my_changes = 0;
@@ -590,20 +642,21 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
       my_changes +=
           wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(a.worklist[w]), k, s, cnt) ? 1u : 0u;
     if ((threadIdx.x & (kWave - 1)) == 0 && my_changes) atomicAdd(&s_changes, my_changes);
-    if (a.prof && (threadIdx.x & (kWave - 1)) == 0 && static_cast<unsigned>(k) < a.prof_sweeps) {
+    if (a.prof && (threadIdx.x & (kWave - 1)) == 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (cnt.ph[q]) atomicMax(&a.prof[static_cast<size_t>(k) * 4 + q], static_cast<unsigned long long>(cnt.ph[q]));
+        atomicMax(&s_ph[q], static_cast<unsigned long long>(cnt.ph[q]));
         cnt.ph[q] = 0;
       }
     }
     __syncthreads();
     const unsigned wg_changes = s_changes;
-    const int done = grid_barrier(a, k, wg_changes, t0, cnt);
+    const int done = grid_barrier(a, k, wg_changes, t0, cnt, s_ph);
     if (threadIdx.x == 0) {
       s_changes = 0;
       t0 = wall_clock64();
     }
+    if (threadIdx.x < 4) s_ph[threadIdx.x] = 0;
     if (done) break;
   }
   flush_counters(a, cnt);
@@ -839,7 +892,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   const size_t max_groups = static_cast<size_t>(dev->cu_count) * 32;
   const size_t o_slots = off; off = align_up(off + max_groups * sizeof(Slot), 256);
   const unsigned prof_sweeps = tn.profile ? 8192u : 0u;
-  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(prof_sweeps) * 4 * sizeof(unsigned long long), 256);
+  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(prof_sweeps) * kProfCols * sizeof(unsigned long long), 256);
   void *ws = nullptr;
   pg_status st = workspace(dev->device, off, &ws);
   if (st != PG_OK) return st;
@@ -854,7 +907,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   a.slots = reinterpret_cast<Slot *>(base + o_slots);
   a.prof = prof_sweeps ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
   a.prof_sweeps = prof_sweeps;
-  if (prof_sweeps) PG_HIP(hipMemsetAsync(a.prof, 0, static_cast<size_t>(prof_sweeps) * 4 * 8, stream));
+  if (prof_sweeps) PG_HIP(hipMemsetAsync(a.prof, 0, static_cast<size_t>(prof_sweeps) * kProfCols * 8, stream));
   a.bits = boundaries;
 
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evr0 = nullptr, evr1 = nullptr;
@@ -978,9 +1031,15 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     stats->lds_bitset = static_cast<uint32_t>(a.lds_bits);
     for (int q = 0; q < 4; ++q) stats->phase_ms[q] = 0.0;
     if (prof_sweeps) {
-      std::vector<unsigned long long> prof(static_cast<size_t>(prof_sweeps) * 4);
+      std::vector<unsigned long long> prof(static_cast<size_t>(prof_sweeps) * kProfCols);
       PG_HIP(hipMemcpy(prof.data(), a.prof, prof.size() * 8, hipMemcpyDeviceToHost));
-      for (size_t r = 0; r < prof.size(); ++r) stats->phase_ms[r % 4] += static_cast<double>(prof[r]) / clock_khz;
+      for (size_t r = 0; r < prof.size(); ++r) {
+        const double ms = static_cast<double>(prof[r]) / clock_khz;
+        if (r % kProfCols < 4)
+          stats->phase_ms[r % kProfCols] += ms;
+        else
+          stats->barrier_ms[r % kProfCols - 4] += ms;
+      }
     }
   }
   if (book[2]) return set_error(PG_ERR_NOT_CONVERGED, "iteration cap reached before convergence");

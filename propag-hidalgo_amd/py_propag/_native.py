@@ -49,14 +49,13 @@ class Stats(C.Structure):
                 ("relax_ms", C.c_double), ("work_ms", C.c_double), ("wait_ms", C.c_double),
                 ("crit_ms", C.c_double),
                 ("workgroups", C.c_uint32), ("lds_bitset", C.c_uint32),
-                ("phase_ms", C.c_double * 4)]
+                ("phase_ms", C.c_double * 4), ("barrier_ms", C.c_double * 4)]
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "phase_ms"}
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("phase_ms", "barrier_ms")}
         d["phase_ms"] = list(self.phase_ms)
+        d["barrier_ms"] = list(self.barrier_ms)
         return d
-
-
 
 
 STATUS = {0: "PG_OK", 1: "PG_ERR_INVALID", 2: "PG_ERR_HIP", 3: "PG_ERR_NO_DEVICE",
