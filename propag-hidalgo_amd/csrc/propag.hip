@@ -46,6 +46,7 @@ constexpr int kTile = 8;                // tile edge (cells): one cell per lane 
 constexpr int kHalo = kTile + 2;        // staged edge
 constexpr int kHaloCells = kHalo * kHalo;
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;  // work-list slot not written yet
 constexpr int kStampNever = INT_MIN / 2;
 constexpr int kWalkBatch = 8;           // DDA cells tested per batch
 constexpr int kWave = 64;               // lanes per wavefront
@@ -58,9 +59,9 @@ struct alignas(128) Line {
 
 // Grid-wide state of one propagation; each hot word sits on its own line.
 struct Control {
-  Line gen;          // [0] barrier generation (sweep + 1 once released)
+  Line gen;          // [0] barrier release: sweep + 1, bit 31 = finished
   Line live;         // [0] worklist length (grows during a sweep)
-  Line book;         // [0] live snapshot, [1] done, [2] capped, [3] error, [4] last sweep
+  Line book;         // [0] live snapshot (plain form), [1] done, [2] capped, [3] error, [4] last sweep
   Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
   unsigned long long stats[8];  // relaxations, improvements, dda_steps, tiles, work, wait,
                                 // critical-path ticks
@@ -75,6 +76,42 @@ __device__ inline void st_relaxed(unsigned *p, unsigned v) {
 __device__ inline unsigned add_relaxed(unsigned *p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Data that other workgroups write during a persistent launch (cell state,
+// stamps, work list, barrier words) is read and written only with these
+// device-coherent accesses (agent-scope relaxed atomics: global loads and
+// stores with sc1, which are neither served from nor left dirty in an XCD's
+// own L2).  A grid barrier then needs no L2 write-back or invalidation, only
+// each wave's wait for its own stores (wait_stores), and read-only data
+// (fire coefficients, the boundary bitset) stays cached across sweeps.
+__device__ inline uint32_t ld_u32(const uint32_t *p) {
+  return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_u32(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_i32(const int *p) {
+  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_i32(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline unsigned long long ld_u64(const unsigned long long *p) {
+  return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_u64(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint2 ld_state(const uint2 *p) {
+  const unsigned long long v = ld_u64(reinterpret_cast<const unsigned long long *>(p));
+  return make_uint2(static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32));
+}
+__device__ inline void st_state(uint2 *p, uint2 v) {
+  st_u64(reinterpret_cast<unsigned long long *>(p),
+         static_cast<unsigned long long>(v.x) | (static_cast<unsigned long long>(v.y) << 32));
+}
+// Returns once every store this wave has issued is complete (for the
+// coherent ones: visible device-wide); the clobber keeps the compiler from
+// moving memory accesses across it.
+__device__ inline void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // ------------------------------------------------------------------ geometry
 
@@ -314,7 +351,7 @@ __device__ inline void grow_worklist(const Args &a, int tx, int ty, int lane) {
         if (__hip_atomic_compare_exchange_strong(&a.tile_live[t], &expected, 1u, __ATOMIC_RELAXED,
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           const unsigned slot = add_relaxed(&a.ctl->live.v[0], 1u);
-          a.worklist[slot] = static_cast<uint32_t>(t);
+          st_u32(&a.worklist[slot], static_cast<uint32_t>(t));
         }
       }
     }
@@ -346,8 +383,8 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     const int gx = ox + h % kHalo - 1, gy = oy + h / kHalo - 1;
     const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
     const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
-    w[it] = inb ? st_in[gi] : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
-    stp[it] = inb ? a.stamp[gi] : kStampNever;
+    w[it] = inb ? ld_state(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
+    stp[it] = inb ? ld_i32(&a.stamp[gi]) : kStampNever;
   }
   float rt[kPer];
   int rstp[kPer];
@@ -359,8 +396,8 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     const bool gather = t < g.max_time && t < FLT_MAX;
     const size_t ri =
         gather ? static_cast<size_t>(ref_x<kCache>(w[it].y)) + static_cast<size_t>(ref_y<kCache>(w[it].y)) * g.W : 0;
-    rt[it] = gather ? __uint_as_float(st_in[ri].x) : FLT_MAX;
-    rstp[it] = gather ? a.stamp[ri] : kStampNever;
+    rt[it] = gather ? __uint_as_float(ld_u32(&st_in[ri].x)) : FLT_MAX;
+    rstp[it] = gather ? ld_i32(&a.stamp[ri]) : kStampNever;
   }
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
@@ -449,11 +486,11 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
                                cnt.steps))
         best_ref |= kClearBit;
       if (a.prof) walk_ticks += wall_clock64() - tw;
-      st_out[gi] = pack_state(best_t, best_ref);
-      a.stamp[gi] = k;
+      st_state(&st_out[gi], pack_state(best_t, best_ref));
+      st_i32(&a.stamp[gi], k);
       ++cnt.improve;
     } else {
-      st_out[gi] = pack_state(own_t, s.ref[me]);
+      st_state(&st_out[gi], pack_state(own_t, s.ref[me]));
     }
   }
   uint64_t tp2 = a.prof ? wall_clock64() : 0;
@@ -477,73 +514,89 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
   return changed;
 }
 
-__device__ inline unsigned long long ld_relaxed64(const unsigned long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier.  Every workgroup publishes its arrival with one release
-// store into its own line; wave 0 of workgroup 0 polls all lines with
-// relaxed loads, does the sweep's bookkeeping (quiet-sweep test, cap,
-// work-list snapshot, critical-path time) and releases the generation word,
-// which the other workgroups poll with relaxed loads.  Each side issues one
-// acquire fence after its wait.  Returns 1 when the propagation is finished.
-__device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t sweep_t0,
-                            Counters &cnt, const unsigned long long *wg_ph) {
-  __shared__ unsigned long long s_release;  // master: time of the last release
+// Grid barrier.  Callers have waited for their stores (wait_stores) and
+// passed __syncthreads.  Every workgroup publishes one 64-bit arrival word in
+// its own line: (sweep + 1) << 32 | changed << 31 | work ticks.  Wave 0 of
+// workgroup 0 polls all lines, a batch of loads at a time, folds the
+// quiet-sweep test, the iteration cap and the critical-path time, and
+// releases the generation word, (sweep + 1) | finished << 31, which thread 0
+// of every other workgroup polls.  All words are device-coherent, so no
+// cache maintenance is needed.  Returns 1 when the propagation is finished.
+__device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t sweep_t0, Counters &cnt,
+                            const unsigned long long *wg_ph) {
   __shared__ int s_done;
-  __syncthreads();
-  const unsigned long long target = static_cast<unsigned long long>(k) + 1ull;
+  __shared__ unsigned long long s_release;  // master: time of the last release
+  const unsigned target = static_cast<unsigned>(k) + 1u;
   Slot *slots = a.slots;
   Control *c = a.ctl;
   uint64_t t_arrive = 0;
   if (threadIdx.x == 0) {
     t_arrive = wall_clock64();
-    cnt.work_ticks += t_arrive - sweep_t0;
-    __hip_atomic_store(&slots[blockIdx.x].work, static_cast<unsigned long long>(t_arrive - sweep_t0),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t work = t_arrive - sweep_t0;
+    cnt.work_ticks += work;
     if (a.prof) {
-      for (int q = 0; q < 4; ++q) slots[blockIdx.x].ph[q] = wg_ph[q];
-      slots[blockIdx.x].t_seen = sweep_t0;
-      slots[blockIdx.x].t_arrive = t_arrive;
+      for (int q = 0; q < 4; ++q) st_u64(&slots[blockIdx.x].ph[q], wg_ph[q]);
+      st_u64(&slots[blockIdx.x].t_seen, sweep_t0);
+      st_u64(&slots[blockIdx.x].t_arrive, t_arrive);
+      wait_stores();
     }
-    __hip_atomic_store(&slots[blockIdx.x].arrive, (target << 32) | wg_changes, __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t w31 = work < 0x7fffffffull ? work : 0x7fffffffull;
+    st_u64(&slots[blockIdx.x].arrive,
+           (static_cast<unsigned long long>(target) << 32) | (wg_changed ? (1ull << 31) : 0ull) | w31);
   }
   if (blockIdx.x == 0) {
     if (threadIdx.x < kWave) {
-      const int lane = threadIdx.x;
-      unsigned long long changes = 0, maxwork = 0;
-      bool timeout = false;
-      for (unsigned i = lane; i < gridDim.x; i += kWave) {
-        unsigned spins = 0;
-        unsigned long long v;
-        while (((v = ld_relaxed64(&slots[i].arrive)) >> 32) != target) {
+      const unsigned lane = threadIdx.x;
+      const unsigned G = gridDim.x;
+      const unsigned nq = (G + kWave - 1) / kWave;  // slots per lane (at most 64)
+      unsigned long long pending = 0;
+      for (unsigned q = 0; q < nq; ++q)
+        if (lane + q * kWave < G) pending |= 1ull << q;
+      bool changed = false, timeout = false;
+      unsigned long long maxwork = 0;
+      unsigned spins = 0;
+      while (pending) {
+        for (unsigned q0 = 0; q0 < nq; q0 += 4) {
+          unsigned long long v[4];
+#pragma unroll
+          for (unsigned j = 0; j < 4; ++j) {
+            const unsigned q = q0 + j;
+            v[j] = (q < nq && ((pending >> q) & 1ull)) ? ld_u64(&slots[lane + q * kWave].arrive) : 0ull;
+          }
+#pragma unroll
+          for (unsigned j = 0; j < 4; ++j) {
+            const unsigned q = q0 + j;
+            if (q < nq && ((pending >> q) & 1ull) && (v[j] >> 32) == target) {
+              pending &= ~(1ull << q);
+              changed = changed || ((v[j] >> 31) & 1ull);
+              const unsigned long long w = v[j] & 0x7fffffffull;
+              maxwork = w > maxwork ? w : maxwork;
+            }
+          }
+        }
+        if (pending) {
           __builtin_amdgcn_s_sleep(1);
           if (++spins > a.spin_limit) {
             timeout = true;
             break;
           }
         }
-        changes += v & 0xffffffffull;
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      for (unsigned i = lane; i < gridDim.x; i += kWave) {
-        const unsigned long long w = ld_relaxed64(&slots[i].work);
-        maxwork = w > maxwork ? w : maxwork;
-      }
+      asm volatile("" ::: "memory");
       for (int off = 32; off > 0; off >>= 1) {
-        changes += __shfl_xor(changes, off);
         const unsigned long long o = __shfl_xor(maxwork, off);
         maxwork = o > maxwork ? o : maxwork;
       }
+      changed = __ballot(changed) != 0ull;
+      timeout = __ballot(timeout) != 0ull;
       if (a.prof && static_cast<unsigned>(k) < a.prof_sweeps) {
         const unsigned long long t_all = wall_clock64();
         // k == 0: the kernel start of workgroup 0 stands in for a release
         const unsigned long long rel = k == 0 ? __shfl(sweep_t0, 0) : s_release;
         unsigned long long m[7] = {0, 0, 0, 0, 0, 0, 0};  // ph[4], start, span, arrival
-        for (unsigned i = lane; i < gridDim.x; i += kWave) {
-          for (int q = 0; q < 4; ++q) m[q] = max(m[q], slots[i].ph[q]);
-          const unsigned long long ts = slots[i].t_seen, ta = slots[i].t_arrive;
+        for (unsigned i = lane; i < G; i += kWave) {
+          for (int q = 0; q < 4; ++q) m[q] = max(m[q], ld_u64(&slots[i].ph[q]));
+          const unsigned long long ts = ld_u64(&slots[i].t_seen), ta = ld_u64(&slots[i].t_arrive);
           m[4] = max(m[4], ts > rel ? ts - rel : 0ull);
           m[5] = max(m[5], ta - ts);
           m[6] = max(m[6], ta);
@@ -557,15 +610,12 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
           row[7] = t_all;  // completed below with the release time
         }
       }
-      timeout = __ballot(timeout) != 0ull;
       if (lane == 0) {
-        const bool cap = static_cast<unsigned>(k + 1) >= a.max_iters;
-        const bool done = changes == 0ull || cap || timeout;
-        st_relaxed(&c->book.v[0], ld_relaxed(&c->live.v[0]));
-        if (cap && changes != 0ull) st_relaxed(&c->book.v[2], 1u);
+        const bool cap = target >= a.max_iters;
+        const bool done = !changed || cap || timeout;
+        if (cap && changed) st_relaxed(&c->book.v[2], 1u);
         if (timeout) st_relaxed(&c->book.v[3], 1u);
         if (done) st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
-        st_relaxed(&c->book.v[1], done ? 1u : 0u);
         c->stats[6] += maxwork;  // only this lane ever writes it
         if (a.prof) {
           s_release = wall_clock64();
@@ -574,15 +624,14 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
             row[7] = s_release - row[7];
           }
         }
-        __hip_atomic_store(&c->gen.v[0], static_cast<unsigned>(target), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        st_relaxed(&c->gen.v[0], target | (done ? 0x80000000u : 0u));
         s_done = done ? 1 : 0;
       }
     }
   } else if (threadIdx.x == 0) {
-    unsigned spins = 0;
+    unsigned spins = 0, g;
     bool timeout = false;
-    while (ld_relaxed(&c->gen.v[0]) != static_cast<unsigned>(target)) {
+    while (((g = ld_relaxed(&c->gen.v[0])) & 0x7fffffffu) != target) {
       __builtin_amdgcn_s_sleep(1);
       if (++spins > a.spin_limit) {
         timeout = true;
@@ -590,8 +639,8 @@ __device__ int grid_barrier(const Args &a, int k, unsigned wg_changes, uint64_t 
         break;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    s_done = (timeout || ld_relaxed(&c->book.v[1]) != 0u || ld_relaxed(&c->book.v[3]) != 0u) ? 1 : 0;
+    asm volatile("" ::: "memory");
+    s_done = (timeout || (g >> 31) != 0u) ? 1 : 0;
   }
   __syncthreads();
   if (threadIdx.x == 0) cnt.wait_ticks += wall_clock64() - t_arrive;
@@ -610,20 +659,30 @@ __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   }
 }
 
-// Persistent form: one launch runs every sweep.  Each wave strides over the
-// live work list, one tile at a time; workgroups meet at the grid barrier
-// after each sweep.  Dynamic LDS: one TileLds per wave, then (optionally)
+// Persistent form: one launch runs every sweep; workgroups meet at the grid
+// barrier after each sweep.  Work-list entry w belongs to wave w / gridDim.x
+// of workgroup w % gridDim.x, so tiles found together (neighbours on the
+// fire front) land on different CUs.  Slots fill in order and never change
+// once written; a wave keeps its first 64 tile ids in one register (lane i:
+// its i-th entry) and each sweep probes the slots after them.  A slot still
+// unwritten when probed belongs to a tile added during this sweep, which
+// this sweep need not visit (visiting it anyway would only recompute
+// unchanged cells).  Dynamic LDS: one TileLds per wave, then (optionally)
 // the boundary bitset.
 template <bool kCache, bool kLdsBits>
 __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
-  __shared__ unsigned s_changes;
-  __shared__ unsigned long long s_ph[4];
+  // Per-sweep workgroup flags, double-buffered by sweep parity: sweep k
+  // writes [k & 1], thread 0 clears it after barrier k, and it is written
+  // again only in sweep k + 2, after barrier k + 1.
+  __shared__ unsigned s_changed[2];
+  __shared__ unsigned long long s_ph[2][4];
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
+  const unsigned lane = threadIdx.x & (kWave - 1);
   TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds);
   TileLds &s = slices[wave];
   const unsigned lds_off = nw * static_cast<unsigned>(sizeof(TileLds) / 4);
-  if (threadIdx.x == 0) s_changes = 0;
-  if (threadIdx.x < 4) s_ph[threadIdx.x] = 0;
+  if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
+  if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   if (kLdsBits) {
     const unsigned nwords = (static_cast<unsigned>(a.g.W) * static_cast<unsigned>(a.g.H) + 31u) / 32u;
     for (unsigned i = threadIdx.x; i < nwords; i += blockDim.x) dyn_lds[lds_off + i] = a.bits[i];
@@ -631,32 +690,53 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   __syncthreads();
   Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
-  // Work-list entry w goes to wave w / gridDim.x of workgroup w % gridDim.x:
-  // tiles found together (neighbours on the fire front) land on different
-  // CUs instead of queueing on one CU's SIMDs.
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
+  const size_t ntiles = static_cast<size_t>(a.ntx) * a.nty;
+  uint32_t ids = kNoTile;  // lane i: this wave's i-th work-list entry
+  unsigned known = 0;      // entries found so far (at most 64 cached)
   for (int k = 0;; ++k) {
-    const unsigned n_live = ld_relaxed(&a.ctl->book.v[0]);
-    unsigned my_changes = 0;
-    for (unsigned w = gwave; w < n_live; w += nwaves)
-      my_changes +=
-          wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(a.worklist[w]), k, s, cnt) ? 1u : 0u;
-    if ((threadIdx.x & (kWave - 1)) == 0 && my_changes) atomicAdd(&s_changes, my_changes);
-    if (a.prof && (threadIdx.x & (kWave - 1)) == 0) {
+    uint32_t probe = kNoTile;
+    const size_t wl = gwave + static_cast<size_t>(lane) * nwaves;
+    if (lane >= known && wl < ntiles) probe = ld_u32(&a.worklist[wl]);
+    bool changed = false;
+    for (unsigned i = 0; i < known; ++i)
+      changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(__shfl(ids, static_cast<int>(i))), k, s,
+                                             cnt);
+    if (known < 64u) {
+      // new entries: the run of written slots right after the known ones
+      const unsigned long long r = ~(__ballot(probe != kNoTile) >> known);
+      unsigned nnew = r ? static_cast<unsigned>(__builtin_ctzll(r)) : 64u - known;
+      if (nnew > 64u - known) nnew = 64u - known;
+      if (lane >= known && lane < known + nnew) ids = probe;
+      for (unsigned i = known; i < known + nnew; ++i)
+        changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(__shfl(ids, static_cast<int>(i))), k,
+                                               s, cnt);
+      known += nnew;
+    }
+    if (known == 64u) {
+      // entries beyond the cached ones are read every sweep
+      for (size_t w = gwave + 64ull * nwaves; w < ntiles; w += nwaves) {
+        const uint32_t t = ld_u32(&a.worklist[w]);
+        if (t == kNoTile) break;
+        changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(t), k, s, cnt);
+      }
+    }
+    if (lane == 0 && changed) s_changed[k & 1] = 1u;
+    if (a.prof && lane == 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        atomicMax(&s_ph[q], static_cast<unsigned long long>(cnt.ph[q]));
+        atomicMax(&s_ph[k & 1][q], static_cast<unsigned long long>(cnt.ph[q]));
         cnt.ph[q] = 0;
       }
     }
+    wait_stores();
     __syncthreads();
-    const unsigned wg_changes = s_changes;
-    const int done = grid_barrier(a, k, wg_changes, t0, cnt, s_ph);
+    const int done = grid_barrier(a, k, s_changed[k & 1] != 0u, t0, cnt, s_ph[k & 1]);
     if (threadIdx.x == 0) {
-      s_changes = 0;
+      s_changed[k & 1] = 0;
       t0 = wall_clock64();
     }
-    if (threadIdx.x < 4) s_ph[threadIdx.x] = 0;
+    if (threadIdx.x < 4) s_ph[k & 1][threadIdx.x] = 0;
     if (done) break;
   }
   flush_counters(a, cnt);
@@ -931,6 +1011,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     return dim3(static_cast<unsigned>(b ? b : 1));
   };
   hipLaunchKernelGGL(fill_u32, grid_for(ntiles), dim3(256), 0, stream, a.tile_live, ntiles, 0u);
+  hipLaunchKernelGGL(fill_u32, grid_for(ntiles), dim3(256), 0, stream, a.worklist, ntiles, kNoTile);
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
                      sizeof(Control) / 4, 0u);
   hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
@@ -989,6 +1070,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     if (per_cu < 1) return set_error(PG_ERR_INVALID, "persistent kernel does not fit on a CU");
     unsigned groups = cu * static_cast<unsigned>(per_cu);
     if (groups > max_groups) groups = static_cast<unsigned>(max_groups);
+    if (groups > 64u * kWave) groups = 64u * kWave;  // the barrier master polls <= 64 slots per lane
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
     launched_groups = groups;
     void *kargs[] = {&a};
@@ -1029,7 +1111,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     stats->crit_ms = static_cast<double>(counters[6]) / clock_khz;
     stats->workgroups = launched_groups;
     stats->lds_bitset = static_cast<uint32_t>(a.lds_bits);
-    for (int q = 0; q < 4; ++q) stats->phase_ms[q] = 0.0;
+    for (int q = 0; q < 4; ++q) stats->phase_ms[q] = stats->barrier_ms[q] = 0.0;
     if (prof_sweeps) {
       std::vector<unsigned long long> prof(static_cast<size_t>(prof_sweeps) * kProfCols);
       PG_HIP(hipMemcpy(prof.data(), a.prof, prof.size() * 8, hipMemcpyDeviceToHost));
