@@ -481,10 +481,16 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     improved = best_t < own_t;  // update_point (propag.cu:192-200)
     if (improved) {
       const uint64_t tw = a.prof ? wall_clock64() : 0;
-      if (kCache &&
-          line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), a.bits, lds_off, g.W,
-                               cnt.steps))
-        best_ref |= kClearBit;
+      if (kCache) {
+        // The flag depends only on (cell, reference): an improvement that
+        // keeps the reference keeps its flag.
+        const uint32_t old = s.ref[me];
+        if ((old & ~kClearBit) == best_ref)
+          best_ref = old;
+        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), a.bits, lds_off, g.W,
+                                      cnt.steps))
+          best_ref |= kClearBit;
+      }
       if (a.prof) walk_ticks += wall_clock64() - tw;
       st_state(&st_out[gi], pack_state(best_t, best_ref));
       st_i32(&a.stamp[gi], k);
