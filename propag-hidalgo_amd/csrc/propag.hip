@@ -17,9 +17,19 @@
 //     committed nothing, so it would commit nothing again.  A cell that
 //     changed in sweep k-1 is always re-evaluated and rewritten, which keeps
 //     both buffers equal wherever a cell is skipped;
-//   * only tiles that hold or touch a burned cell are visited (an append-only
-//     worklist; a tile joins when a neighbour tile first changes).
-// Each tile of 16x16 cells is staged with its 1-cell halo in LDS (time,
+//   * only tiles that can hold such a cell are visited.  A tile in which a
+//     cell changed in sweep k marks, for sweep k+1, its 3x3 tile
+//     neighbourhood and every tile within one tile of a cell that takes its
+//     reference from this tile from two or more cells away (each tile keeps
+//     the tile box of those dependents).  A reference at most one cell away
+//     from its dependent lies in the dependent's tile neighbourhood, and a
+//     cell that adopts a reference in sweep k is itself a change of sweep k,
+//     so the marks cover every cell the rule above re-evaluates.
+// Marks are bits of a per-sweep bitmap (double-buffered by sweep parity,
+// cleared by the wave that reads them).  Several scenarios can share one
+// launch: each owns a range of bitmap words and its own rasters, starts at
+// its own sweep, and drops out when a sweep of it commits nothing.
+// Each tile of 8x8 cells is staged with its 1-cell halo in LDS (time,
 // reference, the reference's time, change flags).  The boundary bitset used
 // by the DDA line-of-sight walks lives in LDS when it fits, else in L2.
 //
@@ -30,6 +40,7 @@
 // double (own Cody-Waite + Taylor sin/cos) and rounded to f32.  Every step is
 // an IEEE add/mul/div/sqrt compiled without contraction, so results match the
 // CPU oracle bit for bit.
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -38,15 +49,12 @@
 #include <vector>
 
 #include "pg_internal.h"
+#include "propag_dev.h"
 
 namespace pg {
 namespace {
 
-constexpr int kTile = 8;                // tile edge (cells): one cell per lane of a wave
-constexpr int kHalo = kTile + 2;        // staged edge
-constexpr int kHaloCells = kHalo * kHalo;
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
-constexpr uint32_t kNoTile = 0xFFFFFFFFu;  // work-list slot not written yet
 constexpr int kStampNever = INT_MIN / 2;
 constexpr int kWalkBatch = 8;           // DDA cells tested per batch
 constexpr int kWave = 64;               // lanes per wavefront
@@ -57,11 +65,10 @@ struct alignas(128) Line {
   unsigned v[32];
 };
 
-// Grid-wide state of one propagation; each hot word sits on its own line.
+// Grid-wide state of one launch; each hot word sits on its own line.
 struct Control {
   Line gen;          // [0] barrier release: sweep + 1, bit 31 = finished
-  Line live;         // [0] worklist length (grows during a sweep)
-  Line book;         // [0] live snapshot (plain form), [1] done, [2] capped, [3] error, [4] last sweep
+  Line book;         // [1] done, [3] error bits, [4] last sweep run
   Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
   unsigned long long stats[8];  // relaxations, improvements, dda_steps, tiles, work, wait,
                                 // critical-path ticks
@@ -75,6 +82,12 @@ __device__ inline void st_relaxed(unsigned *p, unsigned v) {
 }
 __device__ inline unsigned add_relaxed(unsigned *p, unsigned v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void min_relaxed(uint32_t *p, uint32_t v) {
+  (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void max_relaxed(uint32_t *p, uint32_t v) {
+  (void)__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Data that other workgroups write during a persistent launch (cell state,
@@ -115,11 +128,9 @@ __device__ inline void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "me
 
 // ------------------------------------------------------------------ geometry
 
-struct Geom {
-  int W, H;
-  float dx, dy, cell, max_time;
-  int north_up, square;
-};
+constexpr int kHalo = kTile + 2;        // staged edge
+constexpr int kHaloCells = kHalo * kHalo;
+
 
 // Deterministic double sin/cos: reduction by pi/2 with the two-part constant
 // of fdlibm (k*P1 exact for |k| < 2^20) and Taylor polynomials in r^2 to
@@ -282,26 +293,26 @@ __device__ inline bool line_clear(int x, int y, int rx, int ry, const uint32_t *
 
 __device__ inline uint2 pack_state(float t, uint32_t ref) { return make_uint2(__float_as_uint(t), ref); }
 
+
+struct Slot;
+
 struct Args {
-  Geom g;
-  const float4 *coef;        // A, B, R, null flag
-  uint2 *st[2];              // double-buffered (time bits, reference word)
-  int *stamp;                // last sweep in which the cell changed
-  const uint32_t *bits;      // boundary bitset (global copy)
-  uint32_t *tile_live;       // 1 once a tile is on the worklist
-  uint32_t *worklist;        // live tile ids
+  const Scen *scen;          // nscen descriptors
+  ScenOut *out;
+  TileQueue q;               // tiles to visit, by sweep parity
+  int nscen;
+  int k_begin, k_end;
   Control *ctl;
-  struct Slot *slots;        // grid-barrier arrival lines, one per workgroup
-  int ntx, nty;              // tiles per row / column
-  unsigned max_iters;
+  Slot *slots;               // grid-barrier arrival lines, one per workgroup
   unsigned spin_limit;
-  int lds_bits;              // bitset staged in LDS
-  int cache;                 // line-of-sight flags cached in reference words
-  unsigned long long *prof;  // [sweep][kProfCols] profile ticks, or null
+  int lds_bits;              // bitset of scenario 0 staged in LDS
+  const uint32_t *lds_src;   // that bitset
+  unsigned lds_words;
+  unsigned long long *prof;  // [sweep - k_begin][kProfCols] profile ticks, or null
   unsigned prof_sweeps;      // rows of prof
 };
 
-struct TileLds {  // one wave's staged tile (16x16 + 1-cell halo)
+struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
   float time[kHaloCells];
   uint32_t ref[kHaloCells];
   float rtime[kHaloCells];
@@ -309,8 +320,7 @@ struct TileLds {  // one wave's staged tile (16x16 + 1-cell halo)
 };
 static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
 
-// Per-workgroup arrival slot of the grid barrier, one 128-byte line each:
-// arrive = (sweep + 1) << 32 | changed tiles, work = ticks spent in the sweep.
+// Per-workgroup arrival slot of the grid barrier, one 128-byte line each.
 struct alignas(128) Slot {
   unsigned long long arrive;
   unsigned long long work;
@@ -321,8 +331,8 @@ struct alignas(128) Slot {
 static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
 
 // Profile columns per sweep: slowest wave per phase (staging, update rule,
-// line-of-sight walks, commit and work-list), then the barrier: release of
-// the previous sweep -> last workgroup starting, the longest workgroup span,
+// line-of-sight walks, commit and marks), then the barrier: release of the
+// previous sweep -> last workgroup starting, the longest workgroup span,
 // last arrival -> master has seen all, master bookkeeping.
 constexpr int kProfCols = 8;
 
@@ -340,20 +350,67 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Marks the 8 neighbour tiles of (tx,ty) live, appending newcomers.
-__device__ inline void grow_worklist(const Args &a, int tx, int ty, int lane) {
-  if (lane < 9 && lane != 4) {
-    const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
-    if (nx >= 0 && ny >= 0 && nx < a.ntx && ny < a.nty) {
-      const int t = nx + ny * a.ntx;
-      if (__hip_atomic_load(&a.tile_live[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        unsigned expected = 0u;
-        if (__hip_atomic_compare_exchange_strong(&a.tile_live[t], &expected, 1u, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-          const unsigned slot = add_relaxed(&a.ctl->live.v[0], 1u);
-          st_u32(&a.worklist[slot], static_cast<uint32_t>(t));
-        }
-      }
+constexpr unsigned long long kNoEntry = ~0ull;
+
+// Queues tile (X,Y) of scenario si (descriptor sc) for sweep k1 when `valid`.
+// Called by all lanes of a wave together: each valid lane sets the tile's
+// bit, and the lanes that set it first take consecutive list slots from one
+// atomic add.
+__device__ inline void queue_tiles(const TileQueue &q, const Scen &sc, uint32_t si, int k1, bool valid, int X,
+                                   int Y, int lane) {
+  const bool odd = (k1 & 1) != 0;
+  uint32_t *mark = odd ? q.mark[1] : q.mark[0];
+  bool fresh = false;
+  uint32_t t = 0;
+  if (valid) {
+    t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
+    const uint32_t bit = 1u << (t >> sc.nwords_log2);
+    const uint32_t old = __hip_atomic_fetch_or(&mark[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    fresh = (old & bit) == 0u;
+  }
+  const unsigned long long m = __ballot(fresh);
+  if (!m) return;
+  const int leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = add_relaxed(odd ? q.count[1] : q.count[0], static_cast<unsigned>(__builtin_popcountll(m)));
+  base = __shfl(base, leader);
+  if (fresh) {
+    const size_t slot = base + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << lane) - 1ull)));
+    unsigned long long *list = odd ? q.list[1] : q.list[0];
+    if (slot < q.cap) st_u64(&list[slot], (static_cast<unsigned long long>(si) << 32) | t);
+  }
+}
+
+// Single-thread form for set-up kernels.
+__device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t si, int k1, int X, int Y) {
+  const bool odd = (k1 & 1) != 0;
+  const uint32_t t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
+  const uint32_t bit = 1u << (t >> sc.nwords_log2);
+  uint32_t *mark = odd ? q.mark[1] : q.mark[0];
+  const uint32_t old = atomicOr(&mark[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit);
+  if (old & bit) return;
+  const size_t slot = atomicAdd(odd ? q.count[1] : q.count[0], 1u);
+  if (slot < q.cap) (odd ? q.list[1] : q.list[0])[slot] = (static_cast<unsigned long long>(si) << 32) | t;
+}
+
+// A tile changed in sweep k: queue for sweep k + 1 its 3x3 tile
+// neighbourhood and the box of its far dependents grown by one tile (bx:
+// x0, y0, x1, y1 in lanes 0..3).
+__device__ inline void queue_next(const Args &a, const Scen &sc, uint32_t si, int k, int tx, int ty, uint32_t bx,
+                                  int lane) {
+  {
+    const int X = tx + lane % 3 - 1, Y = ty + lane / 3 - 1;
+    queue_tiles(a.q, sc, si, k + 1, lane < 9 && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y, lane);
+  }
+  const uint32_t x0 = __shfl(bx, 0), y0 = __shfl(bx, 1), x1 = __shfl(bx, 2), y1 = __shfl(bx, 3);
+  if (x0 <= x1 && y0 <= y1) {  // an empty box is x0 = y0 = 0xFFFFFFFF, x1 = y1 = 0
+    const int X0 = max(static_cast<int>(x0) - 1, 0), Y0 = max(static_cast<int>(y0) - 1, 0);
+    const int X1 = min(static_cast<int>(x1) + 1, sc.ntx - 1), Y1 = min(static_cast<int>(y1) + 1, sc.nty - 1);
+    const int w = X1 - X0 + 1, n = w * (Y1 - Y0 + 1);
+    for (int q0 = 0; q0 < n; q0 += kWave) {
+      const int q = q0 + lane;
+      queue_tiles(a.q, sc, si, k + 1, q < n, X0 + q % w, Y0 + q / w, lane);
     }
   }
 }
@@ -362,14 +419,19 @@ __device__ inline void grow_worklist(const Args &a, int tx, int ty, int lane) {
 // the 8x8 tile.  Returns, uniformly over the wave, whether any cell
 // committed an improvement.
 template <bool kCache, bool kLdsBits>
-__device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, TileLds &s, Counters &cnt) {
-  const Geom &g = a.g;
-  const uint2 *__restrict__ st_in = a.st[k & 1];
-  uint2 *__restrict__ st_out = a.st[(k + 1) & 1];
-  const int tx = tile % a.ntx, ty = tile / a.ntx;
+__device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned lds_off, int tile, int k, TileLds &s,
+                          Counters &cnt) {
+  const Geom &g = sc.g;
+  // (selects, not sc.st[k & 1]: a dynamic index would put sc in scratch)
+  const uint2 *__restrict__ st_in = (k & 1) ? sc.st[1] : sc.st[0];
+  uint2 *__restrict__ st_out = (k & 1) ? sc.st[0] : sc.st[1];
+  const int tx = tile % sc.ntx, ty = tile / sc.ntx;
   const int ox = tx * kTile, oy = ty * kTile;
   const int lane = threadIdx.x & (kWave - 1);
   uint64_t tp0 = a.prof ? wall_clock64() : 0;
+
+  // The tile's dependents box (used only if the tile changes).
+  const uint32_t bx = lane < 4 ? ld_u32(reinterpret_cast<const uint32_t *>(&sc.box[tile]) + lane) : 0u;
 
   // Stage the halo tile (load_points_into_shared_memory, propag.cu:231-261)
   // in two rounds of independent loads: the cells' state words and stamps,
@@ -384,7 +446,7 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
     const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
     w[it] = inb ? ld_state(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
-    stp[it] = inb ? ld_i32(&a.stamp[gi]) : kStampNever;
+    stp[it] = inb ? ld_i32(&sc.stamp[gi]) : kStampNever;
   }
   float rt[kPer];
   int rstp[kPer];
@@ -397,7 +459,7 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     const size_t ri =
         gather ? static_cast<size_t>(ref_x<kCache>(w[it].y)) + static_cast<size_t>(ref_y<kCache>(w[it].y)) * g.W : 0;
     rt[it] = gather ? __uint_as_float(ld_u32(&st_in[ri].x)) : FLT_MAX;
-    rstp[it] = gather ? ld_i32(&a.stamp[ri]) : kStampNever;
+    rstp[it] = gather ? ld_i32(&sc.stamp[ri]) : kStampNever;
   }
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
@@ -423,6 +485,8 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
       for (int i = -1; i < 2; ++i) need = need || s.chg[(lx + i) + (ly + j) * kHalo];
   }
   bool improved = false;
+  bool far_dep = false;  // improved with a new reference two or more cells away
+  uint32_t rtile = 0;    // that reference's tile
   if (need) {
     ++cnt.relax;
     float4 win[9];  // fire coefficients of the 3x3 neighbourhood
@@ -432,7 +496,7 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
       for (int i = -1; i < 2; ++i) {
         const int X = gx + i, Y = gy + j;
         win[(j + 1) * 3 + (i + 1)] = (X >= 0 && Y >= 0 && X < g.W && Y < g.H)
-                                         ? a.coef[static_cast<size_t>(X) + static_cast<size_t>(Y) * g.W]
+                                         ? sc.coef[static_cast<size_t>(X) + static_cast<size_t>(Y) * g.W]
                                          : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
       }
     }
@@ -464,7 +528,7 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
           if (kCache)
             blocked = (ref & kClearBit) == 0u;
           else
-            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, a.bits, lds_off, g.W, cnt.steps);
+            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, sc.bits, lds_off, g.W, cnt.steps);
           const uint32_t cand_ref = blocked ? make_ref(nx, ny) : make_ref(rfx, rfy);
           const float cand_t = blocked ? nt : ct;
           if (cand_t < best_t) {
@@ -481,19 +545,23 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
     improved = best_t < own_t;  // update_point (propag.cu:192-200)
     if (improved) {
       const uint64_t tw = a.prof ? wall_clock64() : 0;
+      const uint32_t old = s.ref[me];
+      const bool same_ref = (old & (kCache ? ~kClearBit : ~0u)) == best_ref;
       if (kCache) {
         // The flag depends only on (cell, reference): an improvement that
         // keeps the reference keeps its flag.
-        const uint32_t old = s.ref[me];
-        if ((old & ~kClearBit) == best_ref)
+        if (same_ref)
           best_ref = old;
-        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), a.bits, lds_off, g.W,
+        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), sc.bits, lds_off, g.W,
                                       cnt.steps))
           best_ref |= kClearBit;
       }
       if (a.prof) walk_ticks += wall_clock64() - tw;
+      const int brx = ref_x<kCache>(best_ref), bry = ref_y<kCache>(best_ref);
+      far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
+      rtile = static_cast<uint32_t>(brx / kTile) + static_cast<uint32_t>(bry / kTile) * static_cast<uint32_t>(sc.ntx);
       st_state(&st_out[gi], pack_state(best_t, best_ref));
-      st_i32(&a.stamp[gi], k);
+      st_i32(&sc.stamp[gi], k);
       ++cnt.improve;
     } else {
       st_state(&st_out[gi], pack_state(own_t, s.ref[me]));
@@ -501,7 +569,24 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
   }
   uint64_t tp2 = a.prof ? wall_clock64() : 0;
   const bool changed = __ballot(improved) != 0ull;
-  if (changed) grow_worklist(a, tx, ty, lane);
+  if (changed) {
+    queue_next(a, sc, si, k, tx, ty, bx, lane);
+    // Register this tile as a dependent of each new far reference's tile,
+    // one set of atomics per distinct reference tile.
+    unsigned long long m = __ballot(far_dep);
+    while (m) {
+      const int l = __builtin_ctzll(m);
+      const uint32_t rt0 = __shfl(rtile, l);
+      m &= ~__ballot(far_dep && rtile == rt0);
+      if (lane == l) {
+        uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt0]);
+        min_relaxed(b + 0, static_cast<uint32_t>(tx));
+        min_relaxed(b + 1, static_cast<uint32_t>(ty));
+        max_relaxed(b + 2, static_cast<uint32_t>(tx));
+        max_relaxed(b + 3, static_cast<uint32_t>(ty));
+      }
+    }
+  }
   if (lane == 0) ++cnt.tiles;
   wave_sync();  // the next tile reuses this wave's LDS slice
   if (a.prof) {
@@ -520,19 +605,67 @@ __device__ bool wave_tile(const Args &a, unsigned lds_off, int tile, int k, Tile
   return changed;
 }
 
+// One sweep's share of one wave: list entries gwave, gwave + nwaves, ...
+// (64 per round, one per lane), each slot emptied and its tile's mark
+// cleared after reading.  Entries fill slots from 0, so the first empty slot
+// ends the wave's share.  Tiles of a scenario past its sweep budget are
+// skipped and the scenario flagged as capped.  Returns whether any tile of
+// this wave changed.
+template <bool kCache, bool kLdsBits>
+__device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds &s,
+                           Counters &cnt) {
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const bool odd = (k & 1) != 0;
+  unsigned long long *list = odd ? a.q.list[1] : a.q.list[0];
+  uint32_t *mark = odd ? a.q.mark[1] : a.q.mark[0];
+  bool changed = false;
+  for (size_t e0 = gwave; e0 < a.q.cap; e0 += static_cast<size_t>(nwaves) * kWave) {
+    const size_t e = e0 + static_cast<size_t>(lane) * nwaves;
+    unsigned long long v = kNoEntry;
+    if (e < a.q.cap) v = ld_u64(&list[e]);
+    if (v != kNoEntry) st_u64(&list[e], kNoEntry);  // written again only two sweeps later
+    const unsigned long long m = __ballot(v != kNoEntry);
+    for (unsigned long long r = m; r;) {
+      const int l = __builtin_ctzll(r);
+      r &= r - 1;
+      const unsigned long long ev = __shfl(v, l);
+      const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
+      const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev));
+      if (si >= static_cast<uint32_t>(a.nscen)) continue;
+      const Scen &sc = a.scen[si];
+      if (lane == 0) {
+        const uint32_t bit = 1u << (tile >> sc.nwords_log2);
+        (void)__hip_atomic_fetch_and(&mark[sc.wbase + (tile & ((1u << sc.nwords_log2) - 1u))], ~bit,
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (static_cast<unsigned>(k - sc.k0) >= sc.max_iters) {
+        if (lane == 0) st_u32(&a.out[si].capped, 1u);
+        continue;
+      }
+      if (tile >= static_cast<uint32_t>(sc.ntx) * static_cast<uint32_t>(sc.nty)) continue;
+      const bool tc = wave_tile<kCache, kLdsBits>(a, sc, si, lds_off, static_cast<int>(tile), k, s, cnt);
+      if (tc && lane == 0) st_i32(&a.out[si].last_change, k);
+      changed |= tc;
+    }
+    if (m != ~0ull) break;
+  }
+  return changed;
+}
+
 // Grid barrier.  Callers have waited for their stores (wait_stores) and
 // passed __syncthreads.  Every workgroup publishes one 64-bit arrival word in
 // its own line: (sweep + 1) << 32 | changed << 31 | work ticks.  Wave 0 of
 // workgroup 0 polls all lines, a batch of loads at a time, folds the
-// quiet-sweep test, the iteration cap and the critical-path time, and
+// quiet-sweep test, the end of the launch and the critical-path time, and
 // releases the generation word, (sweep + 1) | finished << 31, which thread 0
 // of every other workgroup polls.  All words are device-coherent, so no
-// cache maintenance is needed.  Returns 1 when the propagation is finished.
+// cache maintenance is needed.  Returns 1 when the launch is finished.
 __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t sweep_t0, Counters &cnt,
                             const unsigned long long *wg_ph) {
   __shared__ int s_done;
   __shared__ unsigned long long s_release;  // master: time of the last release
-  const unsigned target = static_cast<unsigned>(k) + 1u;
+  const unsigned target = static_cast<unsigned>(k - a.k_begin) + 1u;
+  const unsigned row = static_cast<unsigned>(k - a.k_begin);
   Slot *slots = a.slots;
   Control *c = a.ctl;
   uint64_t t_arrive = 0;
@@ -595,10 +728,10 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
       }
       changed = __ballot(changed) != 0ull;
       timeout = __ballot(timeout) != 0ull;
-      if (a.prof && static_cast<unsigned>(k) < a.prof_sweeps) {
+      if (a.prof && row < a.prof_sweeps) {
         const unsigned long long t_all = wall_clock64();
-        // k == 0: the kernel start of workgroup 0 stands in for a release
-        const unsigned long long rel = k == 0 ? __shfl(sweep_t0, 0) : s_release;
+        // first sweep: the kernel start of workgroup 0 stands in for a release
+        const unsigned long long rel = row == 0 ? __shfl(sweep_t0, 0) : s_release;
         unsigned long long m[7] = {0, 0, 0, 0, 0, 0, 0};  // ph[4], start, span, arrival
         for (unsigned i = lane; i < G; i += kWave) {
           for (int q = 0; q < 4; ++q) m[q] = max(m[q], ld_u64(&slots[i].ph[q]));
@@ -610,24 +743,28 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
         for (int q = 0; q < 7; ++q)
           for (int off = 32; off > 0; off >>= 1) m[q] = max(m[q], __shfl_xor(m[q], off));
         if (lane == 0) {
-          unsigned long long *row = a.prof + static_cast<size_t>(k) * kProfCols;
-          for (int q = 0; q < 6; ++q) row[q] = m[q];
-          row[6] = t_all > m[6] ? t_all - m[6] : 0ull;
-          row[7] = t_all;  // completed below with the release time
+          unsigned long long *r = a.prof + static_cast<size_t>(row) * kProfCols;
+          for (int q = 0; q < 6; ++q) r[q] = m[q];
+          r[6] = t_all > m[6] ? t_all - m[6] : 0ull;
+          r[7] = t_all;  // completed below with the release time
         }
       }
       if (lane == 0) {
-        const bool cap = target >= a.max_iters;
-        const bool done = !changed || cap || timeout;
-        if (cap && changed) st_relaxed(&c->book.v[2], 1u);
+        const bool done = !changed || k + 1 >= a.k_end || timeout;
         if (timeout) st_relaxed(&c->book.v[3], 1u);
-        if (done) st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
+        if (done) {
+          st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
+          st_relaxed(&c->book.v[1], changed ? 0u : 1u);
+        }
         c->stats[6] += maxwork;  // only this lane ever writes it
+        // This sweep's list is consumed; sweep k + 1 appends to it again.
+        st_relaxed((k & 1) ? a.q.count[1] : a.q.count[0], 0u);
+        wait_stores();
         if (a.prof) {
           s_release = wall_clock64();
-          if (static_cast<unsigned>(k) < a.prof_sweeps) {
-            unsigned long long *row = a.prof + static_cast<size_t>(k) * kProfCols;
-            row[7] = s_release - row[7];
+          if (row < a.prof_sweeps) {
+            unsigned long long *r = a.prof + static_cast<size_t>(row) * kProfCols;
+            r[7] = s_release - r[7];
           }
         }
         st_relaxed(&c->gen.v[0], target | (done ? 0x80000000u : 0u));
@@ -665,16 +802,12 @@ __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   }
 }
 
-// Persistent form: one launch runs every sweep; workgroups meet at the grid
-// barrier after each sweep.  Work-list entry w belongs to wave w / gridDim.x
-// of workgroup w % gridDim.x, so tiles found together (neighbours on the
-// fire front) land on different CUs.  Slots fill in order and never change
-// once written; a wave keeps its first 64 tile ids in one register (lane i:
-// its i-th entry) and each sweep probes the slots after them.  A slot still
-// unwritten when probed belongs to a tile added during this sweep, which
-// this sweep need not visit (visiting it anyway would only recompute
-// unchanged cells).  Dynamic LDS: one TileLds per wave, then (optionally)
-// the boundary bitset.
+// Persistent form: one launch runs sweeps k_begin.. until a sweep commits
+// nothing or k_end; workgroups meet at the grid barrier after each sweep.
+// Bitmap word j belongs to wave j mod (waves) with waves numbered
+// wave * gridDim.x + workgroup, so neighbouring tiles (which sit in
+// neighbouring words) land on different CUs.  Dynamic LDS: one TileLds per
+// wave, then (optionally) the boundary bitset.
 template <bool kCache, bool kLdsBits>
 __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   // Per-sweep workgroup flags, double-buffered by sweep parity: sweep k
@@ -690,43 +823,14 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   if (kLdsBits) {
-    const unsigned nwords = (static_cast<unsigned>(a.g.W) * static_cast<unsigned>(a.g.H) + 31u) / 32u;
-    for (unsigned i = threadIdx.x; i < nwords; i += blockDim.x) dyn_lds[lds_off + i] = a.bits[i];
+    for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
   }
   __syncthreads();
   Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
-  const size_t ntiles = static_cast<size_t>(a.ntx) * a.nty;
-  uint32_t ids = kNoTile;  // lane i: this wave's i-th work-list entry
-  unsigned known = 0;      // entries found so far (at most 64 cached)
-  for (int k = 0;; ++k) {
-    uint32_t probe = kNoTile;
-    const size_t wl = gwave + static_cast<size_t>(lane) * nwaves;
-    if (lane >= known && wl < ntiles) probe = ld_u32(&a.worklist[wl]);
-    bool changed = false;
-    for (unsigned i = 0; i < known; ++i)
-      changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(__shfl(ids, static_cast<int>(i))), k, s,
-                                             cnt);
-    if (known < 64u) {
-      // new entries: the run of written slots right after the known ones
-      const unsigned long long r = ~(__ballot(probe != kNoTile) >> known);
-      unsigned nnew = r ? static_cast<unsigned>(__builtin_ctzll(r)) : 64u - known;
-      if (nnew > 64u - known) nnew = 64u - known;
-      if (lane >= known && lane < known + nnew) ids = probe;
-      for (unsigned i = known; i < known + nnew; ++i)
-        changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(__shfl(ids, static_cast<int>(i))), k,
-                                               s, cnt);
-      known += nnew;
-    }
-    if (known == 64u) {
-      // entries beyond the cached ones are read every sweep
-      for (size_t w = gwave + 64ull * nwaves; w < ntiles; w += nwaves) {
-        const uint32_t t = ld_u32(&a.worklist[w]);
-        if (t == kNoTile) break;
-        changed |= wave_tile<kCache, kLdsBits>(a, lds_off, static_cast<int>(t), k, s, cnt);
-      }
-    }
+  for (int k = a.k_begin;; ++k) {
+    const bool changed = sweep_wave<kCache, kLdsBits>(a, gwave, nwaves, lds_off, k, s, cnt);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
@@ -752,19 +856,17 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
 template <bool kCache>
 __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   __shared__ TileLds slices[256 / kWave];
-  __shared__ unsigned s_changes;
+  __shared__ unsigned s_changed;
   const int wave = threadIdx.x / kWave;
-  if (threadIdx.x == 0) s_changes = 0;
+  if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
   Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
-  const unsigned n_live = ld_relaxed(&a.ctl->book.v[0]);
   const unsigned nw = blockDim.x / kWave;
-  unsigned my_changes = 0;
-  for (unsigned w = blockIdx.x * nw + wave; w < n_live; w += gridDim.x * nw)
-    my_changes += wave_tile<kCache, false>(a, 0, static_cast<int>(a.worklist[w]), k, slices[wave], cnt) ? 1u : 0u;
-  if ((threadIdx.x & (kWave - 1)) == 0 && my_changes) atomicAdd(&s_changes, my_changes);
+  const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
+  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], cnt);
+  if ((threadIdx.x & (kWave - 1)) == 0 && changed) s_changed = 1u;
   __syncthreads();
-  if (threadIdx.x == 0 && s_changes) add_relaxed(&a.ctl->changes.v[k & 1], s_changes);
+  if (threadIdx.x == 0 && s_changed) add_relaxed(&a.ctl->changes.v[k & 1], 1u);
   flush_counters(a, cnt);
 }
 
@@ -773,12 +875,13 @@ __global__ void sweep_book(Args a, int k) {
   Control *c = a.ctl;
   const unsigned ch = c->changes.v[k & 1];
   c->changes.v[(k + 1) & 1] = 0u;
-  c->book.v[0] = c->live.v[0];
-  const bool cap = static_cast<unsigned>(k + 1) >= a.max_iters;
-  const bool done = ch == 0u || cap;
-  if (cap && ch != 0u) c->book.v[2] = 1u;
-  if (done) c->book.v[4] = static_cast<unsigned>(k);
-  c->book.v[1] = done ? 1u : 0u;
+  *((k & 1) ? a.q.count[1] : a.q.count[0]) = 0u;
+  const bool done = ch == 0u || k + 1 >= a.k_end;
+  if (done) {
+    c->book.v[4] = static_cast<unsigned>(k);
+    c->book.v[1] = ch == 0u ? 1u : 0u;
+  }
+  c->book.v[0] = done ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------- set-up
@@ -789,13 +892,26 @@ __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
+__global__ void init_boxes(uint4 *box, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    box[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+}
+
+__global__ void init_out(ScenOut *o, int k0) {
+  o->last_change = k0 - 1;
+  o->capped = 0u;
+}
+
 // Fire coefficients, packed double-buffered state, stamps, reference
 // validation, the line-of-sight flags of the initial references (needs the
-// boundary bitset, so it runs after boundaries_kernel) and the initial
-// worklist (tiles holding or touching a burned cell).
-__global__ void prep(Args a, const float *speed, const float *az, const float *ecc,
-                     const float *time, const uint16_t *rx, const uint16_t *ry) {
-  const size_t n = static_cast<size_t>(a.g.W) * a.g.H;
+// boundary bitset, so it runs after boundaries_kernel), the dependents boxes
+// of the initial far references and the marks of the first sweep (tiles
+// holding or touching a burned cell).
+__global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const float *speed, const float *az,
+                     const float *ecc, const float *time, const uint16_t *rx, const uint16_t *ry) {
+  const Geom &g = sc.g;
+  const size_t n = static_cast<size_t>(g.W) * g.H;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const float s = speed[i], e = ecc[i];
@@ -806,39 +922,40 @@ __global__ void prep(Args a, const float *speed, const float *az, const float *e
     k.y = static_cast<float>(static_cast<double>(e) * sn);
     k.z = static_cast<float>(1.0 / (static_cast<double>(s) * (1.0 - static_cast<double>(e))));
     k.w = s == 0.0f ? 1.0f : 0.0f;
-    const_cast<float4 *>(a.coef)[i] = k;
+    const_cast<float4 *>(sc.coef)[i] = k;
     const float t = time[i];
     uint32_t ref = static_cast<uint32_t>(rx[i]) | (static_cast<uint32_t>(ry[i]) << 16);
     const bool burned = t < FLT_MAX;
-    const int x = static_cast<int>(i % a.g.W), y = static_cast<int>(i / a.g.W);
-    if (burned && (rx[i] >= a.g.W || ry[i] >= a.g.H)) {
-      atomicOr(&a.ctl->book.v[3], 2u);  // invalid reference
-      a.st[0][i] = pack_state(FLT_MAX, kNoRef);
-      a.st[1][i] = pack_state(FLT_MAX, kNoRef);
-      a.stamp[i] = kStampNever;
+    const int x = static_cast<int>(i % g.W), y = static_cast<int>(i / g.W);
+    if (burned && (rx[i] >= g.W || ry[i] >= g.H)) {
+      atomicOr(err, 2u);  // invalid reference
+      sc.st[0][i] = pack_state(FLT_MAX, kNoRef);
+      sc.st[1][i] = pack_state(FLT_MAX, kNoRef);
+      sc.stamp[i] = kStampNever;
       continue;
     }
-    if (burned && a.cache) {
+    if (burned && sc.cache) {
       uint32_t steps = 0;
-      if (line_clear<false>(x, y, rx[i], ry[i], a.bits, 0, a.g.W, steps)) ref |= kClearBit;
+      if (line_clear<false>(x, y, rx[i], ry[i], sc.bits, 0, g.W, steps)) ref |= kClearBit;
     }
-    a.st[0][i] = pack_state(t, ref);
-    a.st[1][i] = pack_state(t, ref);
-    a.stamp[i] = burned ? -1 : kStampNever;
+    sc.st[0][i] = pack_state(t, ref);
+    sc.st[1][i] = pack_state(t, ref);
+    sc.stamp[i] = burned ? sc.k0 - 1 : kStampNever;
     if (burned) {
       const int tx = x / kTile, ty = y / kTile;
       for (int q = 0; q < 9; ++q) {
-        const int nx = tx + q % 3 - 1, ny = ty + q / 3 - 1;
-        if (nx < 0 || ny < 0 || nx >= a.ntx || ny >= a.nty) continue;
-        const int t2 = nx + ny * a.ntx;
-        if (atomicCAS(&a.tile_live[t2], 0u, 1u) == 0u) a.worklist[atomicAdd(&a.ctl->live.v[0], 1u)] = t2;
+        const int X = tx + q % 3 - 1, Y = ty + q / 3 - 1;
+        if (X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty) queue_tile(tq, sc, si, sc.k0, X, Y);
+      }
+      if (abs(static_cast<int>(rx[i]) - x) > 1 || abs(static_cast<int>(ry[i]) - y) > 1) {
+        uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]);
+        min_relaxed(b + 0, static_cast<uint32_t>(tx));
+        min_relaxed(b + 1, static_cast<uint32_t>(ty));
+        max_relaxed(b + 2, static_cast<uint32_t>(tx));
+        max_relaxed(b + 3, static_cast<uint32_t>(ty));
       }
     }
   }
-}
-
-__global__ void init_control(Control *c) {
-  c->book.v[0] = c->live.v[0];
 }
 
 // pre_propagate's find_boundaries (propag.cu:345-375): a cell is a boundary
@@ -878,171 +995,231 @@ __global__ __launch_bounds__(256) void boundaries_kernel(Geom g, const float *sp
 }
 
 // post_propagate (propag.cu:110-117) plus unpacking into the caller's rasters.
-__global__ void post(Args a, int final_buf, const float *speed, float *time, uint16_t *rx, uint16_t *ry) {
-  const size_t n = static_cast<size_t>(a.g.W) * a.g.H;
-  const uint2 *st = a.st[final_buf];
+__global__ void post(Scen sc, int final_buf, const float *speed, float *time, uint16_t *rx, uint16_t *ry) {
+  const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+  const uint2 *st = sc.st[final_buf];
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const uint2 w = st[i];
     float t = __uint_as_float(w.x);
     // A cell never reached keeps the caller's references untouched.
     if (t < FLT_MAX) {
-      rx[i] = static_cast<uint16_t>(a.cache ? ref_x<true>(w.y) : ref_x<false>(w.y));
-      ry[i] = static_cast<uint16_t>(a.cache ? ref_y<true>(w.y) : ref_y<false>(w.y));
+      rx[i] = static_cast<uint16_t>(sc.cache ? ref_x<true>(w.y) : ref_x<false>(w.y));
+      ry[i] = static_cast<uint16_t>(sc.cache ? ref_y<true>(w.y) : ref_y<false>(w.y));
     }
-    if (speed[i] == 0.0f || t > a.g.max_time) t = FLT_MAX;
+    if (speed[i] == 0.0f || t > sc.g.max_time) t = FLT_MAX;
     time[i] = t;
   }
 }
 
-// ------------------------------------------------------------- workspace
-
-struct Workspace {
-  void *base = nullptr;
-  size_t bytes = 0;
-};
-std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;  // per device
-
-pg_status workspace(int device, size_t bytes, void **out) {
-  std::lock_guard<std::mutex> lk(g_ws_mutex);
-  if (g_ws.size() <= static_cast<size_t>(device)) g_ws.resize(device + 1);
-  Workspace &w = g_ws[device];
-  if (w.bytes < bytes) {
-    if (w.base) PG_HIP(hipFree(w.base));
-    w.base = nullptr;
-    w.bytes = 0;
-    hipError_t e = hipMalloc(&w.base, bytes);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return set_error(PG_ERR_OUT_OF_MEMORY, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
-    }
-    w.bytes = bytes;
-  }
-  *out = w.base;
-  return PG_OK;
-}
+// ------------------------------------------------------------- workspaces
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+struct DeviceBuffer {
+  void *base = nullptr;
+  size_t bytes = 0;
+};
+
+// Grows *buf to at least `bytes` (contents are not kept).
+pg_status ensure(DeviceBuffer *buf, size_t bytes) {
+  if (buf->bytes >= bytes) return PG_OK;
+  if (buf->base) PG_HIP(hipFree(buf->base));
+  buf->base = nullptr;
+  buf->bytes = 0;
+  hipError_t e = hipMalloc(&buf->base, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(PG_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(bytes) + " bytes failed");
+  }
+  buf->bytes = bytes;
+  return PG_OK;
+}
+
+// Per device: the buffers of the one-scenario entry point and of a launch
+// (control block, barrier slots, profile rows), each behind its own lock.
+struct PerDevice {
+  std::mutex call_mu, launch_mu;
+  DeviceBuffer call, launch;
+};
+std::mutex g_dev_mu;
+std::vector<PerDevice *> g_dev;
+
+PerDevice *per_device(int device) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (g_dev.size() <= static_cast<size_t>(device)) g_dev.resize(device + 1, nullptr);
+  if (!g_dev[device]) g_dev[device] = new PerDevice();
+  return g_dev[device];
+}
+
+constexpr unsigned kProfRows = 8192;
+
 }  // namespace
 
-pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const float *speed,
-                        const float *az, const float *ecc, float *time, uint16_t *refs_x,
-                        uint16_t *refs_y, uint32_t *boundaries, const pg_tuning *tuning,
-                        pg_stats *stats, hipStream_t stream) {
+// ------------------------------------------------------------- scenarios
+
+size_t scen_buffer_bytes(size_t W, size_t H) {
+  const size_t n = W * H;
+  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+  size_t off = 0;
+  off = align_up(off + n * sizeof(float4), 256);
+  off = align_up(off + n * sizeof(uint2), 256);
+  off = align_up(off + n * sizeof(uint2), 256);
+  off = align_up(off + n * sizeof(int), 256);
+  off = align_up(off + ntiles * sizeof(uint4), 256);
+  return off;
+}
+
+ScenBuffers scen_carve(void *base, size_t W, size_t H) {
+  const size_t n = W * H;
+  char *p = static_cast<char *>(base);
+  size_t off = 0;
+  ScenBuffers b;
+  b.coef = reinterpret_cast<float4 *>(p + off); off = align_up(off + n * sizeof(float4), 256);
+  b.st0 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
+  b.st1 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
+  b.stamp = reinterpret_cast<int *>(p + off); off = align_up(off + n * sizeof(int), 256);
+  b.box = reinterpret_cast<uint4 *>(p + off);
+  return b;
+}
+
+pg_status scen_geometry(const Settings *settings, Scen *sc) {
   if (!settings) return set_error(PG_ERR_INVALID, "settings is NULL");
   const uint32_t W = settings->geo_ref.width, H = settings->geo_ref.height;
   if (W == 0 || H == 0) return set_error(PG_ERR_INVALID, "raster has no cells");
   if (W > 0xFFFFu || H > 0xFFFFu)
     return set_error(PG_ERR_INVALID, "raster sides must be below 65536 (16-bit references)");
-  if (!speed || !az || !ecc || !time || !refs_x || !refs_y || !boundaries)
-    return set_error(PG_ERR_INVALID, "null raster pointer");
   // A non-finite max_time would let unburned cells (FLT_MAX, no reference)
   // count as burning neighbours.
-  if (!std::isfinite(settings->max_time))
-    return set_error(PG_ERR_INVALID, "max_time must be finite");
-  pg_tuning tn{};
-  if (tuning) tn = *tuning;
-  if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 8");
-  if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 8");
+  if (!std::isfinite(settings->max_time)) return set_error(PG_ERR_INVALID, "max_time must be finite");
   const float gdx = settings->geo_ref.transform.gt.dx, gdy = settings->geo_ref.transform.gt.dy;
+  *sc = Scen{};
+  sc->g.W = static_cast<int>(W);
+  sc->g.H = static_cast<int>(H);
+  sc->g.dx = gdx;
+  sc->g.dy = gdy;
+  sc->g.cell = fabsf(gdx);
+  sc->g.max_time = settings->max_time;
+  sc->g.north_up = gdy < 0.0f;
+  sc->g.square = fabsf(gdx) == fabsf(gdy);
+  sc->cache = (W <= 0x7fffu && H <= 0x7fffu) ? 1 : 0;
+  sc->ntx = static_cast<int>((W + kTile - 1) / kTile);
+  sc->nty = static_cast<int>((H + kTile - 1) / kTile);
+  sc->nwords_log2 = need_words_log2(static_cast<size_t>(sc->ntx) * sc->nty);
+  sc->max_iters = 64u * (W + H) + 1024u;
+  return PG_OK;
+}
 
-  Args a{};
-  a.g.W = static_cast<int>(W);
-  a.g.H = static_cast<int>(H);
-  a.g.dx = gdx;
-  a.g.dy = gdy;
-  a.g.cell = fabsf(gdx);
-  a.g.max_time = settings->max_time;
-  a.g.north_up = gdy < 0.0f;
-  a.g.square = fabsf(gdx) == fabsf(gdy);
-  a.cache = (W <= 0x7fffu && H <= 0x7fffu) ? 1 : 0;
-  a.ntx = static_cast<int>((W + kTile - 1) / kTile);
-  a.nty = static_cast<int>((H + kTile - 1) / kTile);
-  a.max_iters = tn.max_iterations ? tn.max_iterations : 64u * (W + H) + 1024u;
-  a.spin_limit = 1u << (tn.spin_limit_log2 ? (tn.spin_limit_log2 > 30 ? 30 : tn.spin_limit_log2) : 22);
-  const size_t n = static_cast<size_t>(W) * H;
-  const size_t ntiles = static_cast<size_t>(a.ntx) * a.nty;
+pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
+                       const float *speed, const float *az, const float *ecc, const float *time,
+                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries,
+                       hipStream_t stream) {
+  const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+  const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
   const size_t nwords = (n + 31) / 32;
-
-  // Workspace: coef | st0 | st1 | stamp | tile_live | worklist | control
-  size_t off = 0;
-  const size_t o_coef = off; off = align_up(off + n * sizeof(float4), 256);
-  const size_t o_st0 = off; off = align_up(off + n * sizeof(uint2), 256);
-  const size_t o_st1 = off; off = align_up(off + n * sizeof(uint2), 256);
-  const size_t o_stamp = off; off = align_up(off + n * sizeof(int), 256);
-  const size_t o_live = off; off = align_up(off + ntiles * sizeof(uint32_t), 256);
-  const size_t o_work = off; off = align_up(off + ntiles * sizeof(uint32_t), 256);
-  const size_t o_ctl = off; off = align_up(off + sizeof(Control), 256);
-  const size_t max_groups = static_cast<size_t>(dev->cu_count) * 32;
-  const size_t o_slots = off; off = align_up(off + max_groups * sizeof(Slot), 256);
-  const unsigned prof_sweeps = tn.profile ? 8192u : 0u;
-  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(prof_sweeps) * kProfCols * sizeof(unsigned long long), 256);
-  void *ws = nullptr;
-  pg_status st = workspace(dev->device, off, &ws);
-  if (st != PG_OK) return st;
-  char *base = static_cast<char *>(ws);
-  a.coef = reinterpret_cast<const float4 *>(base + o_coef);
-  a.st[0] = reinterpret_cast<uint2 *>(base + o_st0);
-  a.st[1] = reinterpret_cast<uint2 *>(base + o_st1);
-  a.stamp = reinterpret_cast<int *>(base + o_stamp);
-  a.tile_live = reinterpret_cast<uint32_t *>(base + o_live);
-  a.worklist = reinterpret_cast<uint32_t *>(base + o_work);
-  a.ctl = reinterpret_cast<Control *>(base + o_ctl);
-  a.slots = reinterpret_cast<Slot *>(base + o_slots);
-  a.prof = prof_sweeps ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
-  a.prof_sweeps = prof_sweeps;
-  if (prof_sweeps) PG_HIP(hipMemsetAsync(a.prof, 0, static_cast<size_t>(prof_sweeps) * kProfCols * 8, stream));
-  a.bits = boundaries;
-
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evr0 = nullptr, evr1 = nullptr;
-  PG_HIP(hipEventCreate(&ev0));
-  PG_HIP(hipEventCreate(&ev1));
-  PG_HIP(hipEventCreate(&evr0));
-  PG_HIP(hipEventCreate(&evr1));
-  struct EventGuard {
-    hipEvent_t *e[4];
-    ~EventGuard() {
-      for (hipEvent_t *p : e)
-        if (*p) (void)hipEventDestroy(*p);
-    }
-  } guard{{&ev0, &ev1, &evr0, &evr1}};
-  PG_HIP(hipEventRecord(ev0, stream));
-
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
   auto grid_for = [&](size_t items) {
     size_t b = (items + 255) / 256;
     if (b > cu * 8) b = cu * 8;
     return dim3(static_cast<unsigned>(b ? b : 1));
   };
-  hipLaunchKernelGGL(fill_u32, grid_for(ntiles), dim3(256), 0, stream, a.tile_live, ntiles, 0u);
-  hipLaunchKernelGGL(fill_u32, grid_for(ntiles), dim3(256), 0, stream, a.worklist, ntiles, kNoTile);
+  unsigned *err = &out->err;
+  hipLaunchKernelGGL(init_boxes, grid_for(ntiles), dim3(256), 0, stream, sc.box, ntiles);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, err, static_cast<size_t>(1), 0u);
+  hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, sc.g, speed, az, ecc,
+                     boundaries);
+  hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, sc, si, q, err, speed, az, ecc, time, refs_x,
+                     refs_y);
+  PG_HIP(hipGetLastError());
+  unsigned e = 0;
+  PG_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, stream));
+  PG_HIP(hipStreamSynchronize(stream));
+  hipLaunchKernelGGL(init_out, dim3(1), dim3(1), 0, stream, out, sc.k0);
+  PG_HIP(hipGetLastError());
+  if (e & 2u) return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
+  return PG_OK;
+}
+
+pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,
+                      uint16_t *refs_y, hipStream_t stream) {
+  const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+  size_t b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  hipLaunchKernelGGL(post, dim3(static_cast<unsigned>(b)), dim3(256), 0, stream, sc, final_buf, speed, time, refs_x,
+                     refs_y);
+  PG_HIP(hipGetLastError());
+  return PG_OK;
+}
+
+// ---------------------------------------------------------------- launches
+
+pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tuning, pg_stats *stats,
+                     LaunchResult *res, hipStream_t stream) {
+  pg_tuning tn{};
+  if (tuning) tn = *tuning;
+  if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 8");
+  if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 8");
+  if (L.k_end <= L.k_begin) return set_error(PG_ERR_INVALID, "empty sweep range");
+  PerDevice *pd = per_device(dev->device);
+  std::lock_guard<std::mutex> lk(pd->launch_mu);
+
+  const unsigned cu = static_cast<unsigned>(dev->cu_count);
+  const size_t max_groups = static_cast<size_t>(cu) * 32;
+  const unsigned prof_rows = tn.profile ? kProfRows : 0u;
+  size_t off = 0;
+  const size_t o_ctl = off; off = align_up(off + sizeof(Control), 256);
+  const size_t o_slots = off; off = align_up(off + max_groups * sizeof(Slot), 256);
+  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(kProfRows) * kProfCols * 8, 256);
+  pg_status st = ensure(&pd->launch, off);
+  if (st != PG_OK) return st;
+  char *base = static_cast<char *>(pd->launch.base);
+
+  Args a{};
+  a.scen = L.scen;
+  a.out = L.out;
+  a.q = L.q;
+  a.nscen = L.nscen;
+  a.k_begin = L.k_begin;
+  a.k_end = L.k_end;
+  a.ctl = reinterpret_cast<Control *>(base + o_ctl);
+  a.slots = reinterpret_cast<Slot *>(base + o_slots);
+  a.spin_limit = 1u << (tn.spin_limit_log2 ? (tn.spin_limit_log2 > 30 ? 30 : tn.spin_limit_log2) : 22);
+  a.prof = prof_rows ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
+  a.prof_sweeps = prof_rows;
+
+  auto grid_for = [&](size_t items) {
+    size_t b = (items + 255) / 256;
+    if (b > cu * 8) b = cu * 8;
+    return dim3(static_cast<unsigned>(b ? b : 1));
+  };
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
                      sizeof(Control) / 4, 0u);
   hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
                      reinterpret_cast<uint32_t *>(a.slots), max_groups * sizeof(Slot) / 4, 0u);
-  hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, a.g, speed, az, ecc,
-                     boundaries);
-  hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, a, speed, az, ecc, time, refs_x, refs_y);
-  hipLaunchKernelGGL(init_control, dim3(1), dim3(1), 0, stream, a.ctl);
+  if (prof_rows)
+    hipLaunchKernelGGL(fill_u32, grid_for(static_cast<size_t>(prof_rows) * kProfCols * 2), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t *>(a.prof), static_cast<size_t>(prof_rows) * kProfCols * 2, 0u);
   PG_HIP(hipGetLastError());
 
-  // Validation result before any sweep gathers through the references.
-  unsigned book[8];
-  PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
-  PG_HIP(hipStreamSynchronize(stream));
-  if (book[3] & 2u) {
-    return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
-  }
+  hipEvent_t evr0 = nullptr, evr1 = nullptr;
+  PG_HIP(hipEventCreate(&evr0));
+  PG_HIP(hipEventCreate(&evr1));
+  struct EventGuard {
+    hipEvent_t *e[2];
+    ~EventGuard() {
+      for (hipEvent_t *p : e)
+        if (*p) (void)hipEventDestroy(*p);
+    }
+  } guard{{&evr0, &evr1}};
 
-  PG_HIP(hipEventRecord(evr0, stream));
+  unsigned book[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned launched_groups = 0;
+  PG_HIP(hipEventRecord(evr0, stream));
   if (tn.mode == 1) {
     const dim3 grid(cu * 4);
     launched_groups = grid.x;
-    for (int k = 0;; ++k) {
-      if (a.cache)
+    for (int k = L.k_begin;; ++k) {
+      if (L.cache)
         hipLaunchKernelGGL(propagate_sweep<true>, grid, dim3(256), 0, stream, a, k);
       else
         hipLaunchKernelGGL(propagate_sweep<false>, grid, dim3(256), 0, stream, a, k);
@@ -1050,18 +1227,20 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
       PG_HIP(hipGetLastError());
       PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
       PG_HIP(hipStreamSynchronize(stream));
-      if (book[1]) break;
+      if (book[0]) break;
     }
     PG_HIP(hipEventRecord(evr1, stream));
   } else {
     // Waves per workgroup: as many tile slices as fit beside the bitset when
     // the bitset can live in LDS (one workgroup per CU), else 16 waves.
-    const size_t bits_bytes = nwords * sizeof(uint32_t);
+    const size_t bits_bytes = L.lds_bits_words * sizeof(uint32_t);
     const size_t static_lds = 256;
     int waves = 16;
     a.lds_bits = 0;
-    if (bits_bytes + static_lds + sizeof(TileLds) * 4 <= dev->lds_per_block) {
+    if (L.nscen == 1 && L.lds_bits_src && bits_bytes + static_lds + sizeof(TileLds) * 4 <= dev->lds_per_block) {
       a.lds_bits = 1;
+      a.lds_src = L.lds_bits_src;
+      a.lds_words = static_cast<unsigned>(L.lds_bits_words);
       waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / sizeof(TileLds));
     }
     const int max_waves = 16;  // the kernel's launch bound: 1024 threads
@@ -1069,7 +1248,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     if (waves > max_waves) waves = max_waves;
     const size_t shmem = waves * sizeof(TileLds) + (a.lds_bits ? bits_bytes : 0);
     const int block = waves * kWave;
-    void (*kern)(Args) = a.cache ? (a.lds_bits ? propagate_persistent<true, true> : propagate_persistent<true, false>)
+    void (*kern)(Args) = L.cache ? (a.lds_bits ? propagate_persistent<true, true> : propagate_persistent<true, false>)
                                  : (a.lds_bits ? propagate_persistent<false, true> : propagate_persistent<false, false>);
     int per_cu = 0;
     PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, shmem));
@@ -1080,46 +1259,38 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
     launched_groups = groups;
     void *kargs[] = {&a};
-    PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups),
-                                      dim3(block), kargs, static_cast<unsigned>(shmem), stream));
+    PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups), dim3(block), kargs,
+                                      static_cast<unsigned>(shmem), stream));
     PG_HIP(hipEventRecord(evr1, stream));
     PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
   }
-  if (book[3] & 1u) {
-    return set_error(PG_ERR_TIMEOUT, "a grid barrier wait exceeded its spin limit");
-  }
-  const int last = static_cast<int>(book[4]);
-  hipLaunchKernelGGL(post, grid_for(n), dim3(256), 0, stream, a, (last + 1) & 1, speed, time, refs_x, refs_y);
-  PG_HIP(hipGetLastError());
-  PG_HIP(hipEventRecord(ev1, stream));
+  if (book[3] & 1u) return set_error(PG_ERR_TIMEOUT, "a grid barrier wait exceeded its spin limit");
   unsigned long long counters[8];
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
-  float ms = 0.0f, rms = 0.0f;
-  PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  float rms = 0.0f;
   PG_HIP(hipEventElapsedTime(&rms, evr0, evr1));
+  res->last_sweep = static_cast<int>(book[4]);
+  res->finished = book[1] ? 1 : 0;
   if (stats) {
-    stats->iterations = static_cast<uint64_t>(last) + 1;
-    stats->relaxations = counters[0];
-    stats->improvements = counters[1];
-    stats->dda_steps = counters[2];
-    stats->tiles_visited = counters[3];
-    stats->elapsed_ms = ms;
-    stats->relax_ms = rms;
     int clock_khz = 0;
     if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev->device) != hipSuccess ||
         clock_khz <= 0)
       clock_khz = 100000;  // 100 MHz constant clock
     const double groups_d = launched_groups ? static_cast<double>(launched_groups) : 1.0;
-    stats->work_ms = static_cast<double>(counters[4]) / clock_khz / groups_d;
-    stats->wait_ms = static_cast<double>(counters[5]) / clock_khz / groups_d;
-    stats->crit_ms = static_cast<double>(counters[6]) / clock_khz;
+    stats->relaxations += counters[0];
+    stats->improvements += counters[1];
+    stats->dda_steps += counters[2];
+    stats->tiles_visited += counters[3];
+    stats->relax_ms += rms;
+    stats->work_ms += static_cast<double>(counters[4]) / clock_khz / groups_d;
+    stats->wait_ms += static_cast<double>(counters[5]) / clock_khz / groups_d;
+    stats->crit_ms += static_cast<double>(counters[6]) / clock_khz;
     stats->workgroups = launched_groups;
     stats->lds_bitset = static_cast<uint32_t>(a.lds_bits);
-    for (int q = 0; q < 4; ++q) stats->phase_ms[q] = stats->barrier_ms[q] = 0.0;
-    if (prof_sweeps) {
-      std::vector<unsigned long long> prof(static_cast<size_t>(prof_sweeps) * kProfCols);
+    if (prof_rows) {
+      std::vector<unsigned long long> prof(static_cast<size_t>(prof_rows) * kProfCols);
       PG_HIP(hipMemcpy(prof.data(), a.prof, prof.size() * 8, hipMemcpyDeviceToHost));
       for (size_t r = 0; r < prof.size(); ++r) {
         const double ms = static_cast<double>(prof[r]) / clock_khz;
@@ -1130,7 +1301,110 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
       }
     }
   }
-  if (book[2]) return set_error(PG_ERR_NOT_CONVERGED, "iteration cap reached before convergence");
+  return PG_OK;
+}
+
+// The one-scenario entry point: prepare, run to convergence (or the cap),
+// finish.  Buffers come from a per-device workspace held for the call.
+pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const float *speed, const float *az,
+                        const float *ecc, float *time, uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries,
+                        const pg_tuning *tuning, pg_stats *stats, hipStream_t stream) {
+  Scen sc;
+  pg_status st = scen_geometry(settings, &sc);
+  if (st != PG_OK) return st;
+  if (!speed || !az || !ecc || !time || !refs_x || !refs_y || !boundaries)
+    return set_error(PG_ERR_INVALID, "null raster pointer");
+  pg_tuning tn{};
+  if (tuning) tn = *tuning;
+  if (tn.max_iterations) sc.max_iters = tn.max_iterations;
+  if (sc.max_iters > 0x3fffffffu) return set_error(PG_ERR_INVALID, "max_iterations too large");
+  PerDevice *pd = per_device(dev->device);
+  std::lock_guard<std::mutex> lk(pd->call_mu);
+
+  const size_t W = static_cast<size_t>(sc.g.W), H = static_cast<size_t>(sc.g.H);
+  const uint32_t nwords = 1u << sc.nwords_log2;
+  size_t off = 0;
+  const size_t o_scen = off; off = align_up(off + scen_buffer_bytes(W, H), 256);
+  const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
+  const size_t o_mark = off; off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
+  const size_t o_list = off; off = align_up(off + 2 * ntiles * sizeof(unsigned long long), 256);
+  const size_t o_count = off; off = align_up(off + 256, 256);
+  const size_t o_desc = off; off = align_up(off + sizeof(Scen), 256);
+  const size_t o_out = off; off = align_up(off + sizeof(ScenOut), 256);
+  st = ensure(&pd->call, off);
+  if (st != PG_OK) return st;
+  char *base = static_cast<char *>(pd->call.base);
+  const ScenBuffers b = scen_carve(base + o_scen, W, H);
+  sc.coef = b.coef;
+  sc.st[0] = b.st0;
+  sc.st[1] = b.st1;
+  sc.stamp = b.stamp;
+  sc.box = b.box;
+  sc.bits = boundaries;
+  sc.wbase = 0;
+  sc.k0 = 0;
+  TileQueue q;
+  q.mark[0] = reinterpret_cast<uint32_t *>(base + o_mark);
+  q.mark[1] = q.mark[0] + nwords;
+  q.list[0] = reinterpret_cast<unsigned long long *>(base + o_list);
+  q.list[1] = q.list[0] + ntiles;
+  q.count[0] = reinterpret_cast<uint32_t *>(base + o_count);
+  q.count[1] = q.count[0] + 32;
+  q.cap = ntiles;
+  Scen *d_scen = reinterpret_cast<Scen *>(base + o_desc);
+  ScenOut *d_out = reinterpret_cast<ScenOut *>(base + o_out);
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  PG_HIP(hipEventCreate(&ev0));
+  PG_HIP(hipEventCreate(&ev1));
+  struct EventGuard {
+    hipEvent_t *e[2];
+    ~EventGuard() {
+      for (hipEvent_t *p : e)
+        if (*p) (void)hipEventDestroy(*p);
+    }
+  } guard{{&ev0, &ev1}};
+  PG_HIP(hipEventRecord(ev0, stream));
+  const unsigned cu = static_cast<unsigned>(dev->cu_count);
+  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((4 * ntiles + 255) / 256, cu * 8));
+  hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, q.mark[0], static_cast<size_t>(2 * nwords), 0u);
+  hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
+                     reinterpret_cast<uint32_t *>(q.list[0]), 4 * ntiles, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, q.count[0], static_cast<size_t>(64), 0u);
+  PG_HIP(hipGetLastError());
+  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, stream);
+  if (st != PG_OK) return st;
+  PG_HIP(hipMemcpyAsync(d_scen, &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
+
+  Launch L{};
+  L.scen = d_scen;
+  L.out = d_out;
+  L.nscen = 1;
+  L.q = q;
+  L.k_begin = 0;
+  L.k_end = static_cast<int>(sc.max_iters) + 1;  // the last sweep only detects a cap
+  L.cache = sc.cache;
+  L.lds_bits_src = boundaries;
+  L.lds_bits_words = (W * H + 31) / 32;
+  pg_stats local{};
+  LaunchResult res{};
+  st = run_sweeps(dev, L, &tn, &local, &res, stream);
+  if (st != PG_OK) return st;
+  ScenOut o{};
+  PG_HIP(hipMemcpyAsync(&o, d_out, sizeof(o), hipMemcpyDeviceToHost, stream));
+  PG_HIP(hipStreamSynchronize(stream));
+  st = scen_finish(sc, scen_final_buf(sc, o), speed, time, refs_x, refs_y, stream);
+  if (st != PG_OK) return st;
+  PG_HIP(hipEventRecord(ev1, stream));
+  PG_HIP(hipStreamSynchronize(stream));
+  float ms = 0.0f;
+  PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  if (stats) {
+    *stats = local;
+    stats->iterations = o.capped ? sc.max_iters : static_cast<uint64_t>(o.last_change - sc.k0 + 2);
+    stats->elapsed_ms = ms;
+  }
+  if (o.capped) return set_error(PG_ERR_NOT_CONVERGED, "iteration cap reached before convergence");
   return PG_OK;
 }
 

@@ -1,0 +1,127 @@
+// Device-side layout of a propagation scenario and the launch interface of
+// the propagation kernels (propag.hip), shared with the batching context
+// (context.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "pg_internal.h"
+
+namespace pg {
+
+constexpr int kTile = 8;  // tile edge (cells): one cell per lane of a wave
+
+struct Geom {
+  int W, H;
+  float dx, dy, cell, max_time;
+  int north_up, square;
+};
+
+// One scenario as the kernels read it (read-only during a launch).  Tile t
+// of the scenario is bit (t >> nwords_log2) of bitmap word
+// wbase + (t & (2^nwords_log2 - 1)).
+struct Scen {
+  Geom g;
+  const float4 *coef;    // per cell: A, B, R, null flag
+  uint2 *st[2];          // double-buffered (time bits, reference word)
+  int *stamp;            // last sweep in which the cell changed
+  const uint32_t *bits;  // boundary bitset
+  uint4 *box;            // per tile: tile box (x0, y0, x1, y1) of far dependents
+  int ntx, nty;
+  uint32_t wbase, nwords_log2;
+  int k0;                // sweep number of the scenario's first sweep
+  unsigned max_iters;    // sweeps allowed from k0
+  int cache;             // line-of-sight flags cached in reference words
+  int pad;
+};
+
+// Per-scenario results written by the kernels.
+struct ScenOut {
+  int last_change;  // last sweep in which a cell changed (k0 - 1: none)
+  unsigned capped;  // 1: work remained when max_iters sweeps had run
+  unsigned err;     // set-up: 2 = a burned cell has a reference outside the raster
+  unsigned pad;
+};
+
+// Per-sweep queues of tiles to visit, double-buffered by sweep parity: the
+// tiles marked for sweep k sit in list[k & 1][0 .. count[k & 1]) as
+// scenario << 32 | tile, each once (mark[k & 1] holds one bit per tile of
+// every scenario, in the scenario's word range).  Unused list slots hold
+// ~0.  The queues outlive a launch: a launch that stops at k_end leaves the
+// tiles of sweep k_end queued for the next one.
+struct TileQueue {
+  uint32_t *mark[2];
+  unsigned long long *list[2];
+  uint32_t *count[2];  // each on its own 128-byte line
+  size_t cap;          // list slots per parity (>= tiles of all scenarios)
+};
+
+// Bitmap words a scenario of ntiles tiles uses (a power of two >= ntiles/32).
+inline uint32_t need_words_log2(size_t ntiles) {
+  uint32_t l = 0;
+  while ((static_cast<size_t>(32) << l) < ntiles) ++l;
+  return l;
+}
+
+// Device buffers of one scenario, carved by the caller.
+struct ScenBuffers {
+  float4 *coef;
+  uint2 *st0, *st1;
+  int *stamp;
+  uint4 *box;
+};
+size_t scen_buffer_bytes(size_t width, size_t height);
+ScenBuffers scen_carve(void *base, size_t width, size_t height);
+
+// Fills sc's geometry and tiling from a Settings (buffers and bitmap range
+// are the caller's).
+pg_status scen_geometry(const Settings *settings, Scen *sc);
+
+// Boundaries, coefficients, initial state and first-sweep tiles of scenario
+// number si (start sweep sc.k0): writes `boundaries`, the scenario's
+// rasters, queues in q the tiles of sweep sc.k0 and resets *out.  The
+// scenario's bitmap words must be zero.  Returns PG_ERR_INVALID (after
+// synchronising the stream) if a burned cell has a reference outside the
+// raster.
+pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
+                       const float *speed, const float *az, const float *ecc, const float *time,
+                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries,
+                       hipStream_t stream);
+
+// post_propagate for one scenario from state buffer `final_buf`.
+pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,
+                      uint16_t *refs_y, hipStream_t stream);
+
+// Buffer that holds a finished scenario's final state.
+inline int scen_final_buf(const Scen &sc, const ScenOut &o) {
+  if (o.capped) return static_cast<int>((static_cast<unsigned>(sc.k0) + sc.max_iters) & 1u);
+  return (o.last_change + 2) & 1;
+}
+
+// One launch of sweeps [k_begin, k_end) over nscen scenarios (device arrays
+// scen and out).
+struct Launch {
+  const Scen *scen;
+  ScenOut *out;
+  int nscen;
+  TileQueue q;
+  int k_begin, k_end;
+  int cache;           // all scenarios cache line-of-sight flags
+  const uint32_t *lds_bits_src;  // nscen == 1: the bitset, staged in LDS if it fits
+  size_t lds_bits_words;
+};
+
+struct LaunchResult {
+  int last_sweep;      // last sweep run
+  int finished;        // 1: a sweep committed nothing (all scenarios done)
+};
+
+// Runs the launch (persistent cooperative kernel, or one launch per sweep
+// with tuning->mode == 1) and adds its counters to *stats.
+pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tuning, pg_stats *stats,
+                     LaunchResult *res, hipStream_t stream);
+
+}  // namespace pg
