@@ -206,7 +206,8 @@ def main():
         "cpu_baseline": cpu,
         "propagation": {"sweeps": stats[-1]["iterations"], "relaxations": relax,
                         "improvements": impr, "dda_steps": stats[-1]["dda_steps"],
-                        "tiles_visited": stats[-1]["tiles_visited"], "burned_cells": burned,
+                        "tiles_visited": stats[-1]["tiles_visited"], "tile_marks": stats[-1]["tile_marks"],
+                        "burned_cells": burned,
                         "elapsed_ms": float(np.mean([s["elapsed_ms"] for s in stats])),
                         "work_ms": float(np.mean([s["work_ms"] for s in stats])),
                         "wait_ms": float(np.mean([s["wait_ms"] for s in stats])),

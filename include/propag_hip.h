@@ -89,6 +89,7 @@ typedef struct pg_stats {
                                started, the longest workgroup span, last
                                arrival until the master had seen all arrivals,
                                and the master's bookkeeping                   */
+  uint64_t tile_marks;      /* tiles queued for a next sweep (with repeats)   */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

@@ -74,20 +74,36 @@ struct Control {
                                 // critical-path ticks
 };
 
+// Every pointer the kernels use points to device memory; pointers read from
+// a scenario descriptor are generic to the compiler, which would emit FLAT
+// accesses (counted against the LDS wait counter too).  G() restates them as
+// global.
+#define PG_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline PG_GLOBAL T *G(T *p) {
+  return (PG_GLOBAL T *)(p);
+}
+
 __device__ inline unsigned ld_relaxed(const unsigned *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(G(const_cast<unsigned *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_relaxed(unsigned *p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline unsigned add_relaxed(unsigned *p, unsigned v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_add(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint32_t or_fetch(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_or(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void and_relaxed(uint32_t *p, uint32_t v) {
+  (void)__hip_atomic_fetch_and(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void min_relaxed(uint32_t *p, uint32_t v) {
-  (void)__hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  (void)__hip_atomic_fetch_min(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void max_relaxed(uint32_t *p, uint32_t v) {
-  (void)__hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  (void)__hip_atomic_fetch_max(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Data that other workgroups write during a persistent launch (cell state,
@@ -98,20 +114,22 @@ __device__ inline void max_relaxed(uint32_t *p, uint32_t v) {
 // each wave's wait for its own stores (wait_stores), and read-only data
 // (fire coefficients, the boundary bitset) stays cached across sweeps.
 __device__ inline uint32_t ld_u32(const uint32_t *p) {
-  return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(G(const_cast<uint32_t *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_u32(uint32_t *p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline int ld_i32(const int *p) {
-  return __hip_atomic_load(const_cast<int *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(G(const_cast<int *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ inline void st_i32(int *p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void st_i32(int *p, int v) {
+  __hip_atomic_store(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline unsigned long long ld_u64(const unsigned long long *p) {
-  return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(G(const_cast<unsigned long long *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void st_u64(unsigned long long *p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline uint2 ld_state(const uint2 *p) {
   const unsigned long long v = ld_u64(reinterpret_cast<const unsigned long long *>(p));
@@ -211,7 +229,7 @@ extern __shared__ uint32_t dyn_lds[];
 
 template <bool kLdsBits>
 __device__ inline bool bit_at(const uint32_t *gbits, unsigned lds_off, uint32_t idx) {
-  const uint32_t w = kLdsBits ? dyn_lds[lds_off + (idx >> 5)] : gbits[idx >> 5];
+  const uint32_t w = kLdsBits ? dyn_lds[lds_off + (idx >> 5)] : G(gbits)[idx >> 5];
   return (w >> (idx & 31u)) & 1u;
 }
 
@@ -316,6 +334,7 @@ struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
   float time[kHaloCells];
   uint32_t ref[kHaloCells];
   float rtime[kHaloCells];
+  float ca[kHaloCells], cb[kHaloCells], cr[kHaloCells];  // fire coefficients A, B, R (R = inf: null fire)
   uint8_t chg[kHaloCells];
 };
 static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
@@ -337,7 +356,7 @@ static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
 constexpr int kProfCols = 8;
 
 struct Counters {
-  uint32_t relax, improve, steps, tiles;
+  uint32_t relax, improve, steps, tiles, marks;
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
 };
@@ -352,67 +371,107 @@ __device__ inline void wave_sync() {
 
 constexpr unsigned long long kNoEntry = ~0ull;
 
-// Queues tile (X,Y) of scenario si (descriptor sc) for sweep k1 when `valid`.
-// Called by all lanes of a wave together: each valid lane sets the tile's
-// bit, and the lanes that set it first take consecutive list slots from one
-// atomic add.
-__device__ inline void queue_tiles(const TileQueue &q, const Scen &sc, uint32_t si, int k1, bool valid, int X,
-                                   int Y, int lane) {
-  const bool odd = (k1 & 1) != 0;
-  uint32_t *mark = odd ? q.mark[1] : q.mark[0];
-  bool fresh = false;
-  uint32_t t = 0;
-  if (valid) {
-    t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
-    const uint32_t bit = 1u << (t >> sc.nwords_log2);
-    const uint32_t old = __hip_atomic_fetch_or(&mark[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    fresh = (old & bit) == 0u;
+__device__ inline unsigned long long *list_of(const TileQueue &q, int k) {
+  switch (k & 3) {
+    case 0: return q.list[0];
+    case 1: return q.list[1];
+    case 2: return q.list[2];
+    default: return q.list[3];
   }
+}
+__device__ inline uint32_t *count_of(const TileQueue &q, int k) {
+  switch (k & 3) {
+    case 0: return q.count[0];
+    case 1: return q.count[1];
+    case 2: return q.count[2];
+    default: return q.count[3];
+  }
+}
+__device__ inline uint32_t *mark_of(const TileQueue &q, int k) { return (k & 1) ? q.mark[1] : q.mark[0]; }
+
+// Queuing a tile for sweep k1 is split in two so that the first round trip
+// can overlap other work: mark_issue sets the tile's bit (lanes with
+// `valid`), mark_commit gives the lanes that set it first consecutive list
+// slots from one atomic add.  Both are called by all lanes of a wave.
+struct Mark {
+  uint32_t t, bit, old;
+  bool valid;
+};
+
+__device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, int k1, bool valid, int X, int Y) {
+  Mark m{0u, 0u, 0u, valid};
+  if (valid) {
+    m.t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
+    m.bit = 1u << (m.t >> sc.nwords_log2);
+    m.old = or_fetch(&mark_of(q, k1)[sc.wbase + (m.t & ((1u << sc.nwords_log2) - 1u))], m.bit);
+  }
+  return m;
+}
+
+__device__ inline void mark_commit(const TileQueue &q, uint32_t si, int k1, const Mark &mk, int lane) {
+  const bool fresh = mk.valid && (mk.old & mk.bit) == 0u;
   const unsigned long long m = __ballot(fresh);
   if (!m) return;
   const int leader = __builtin_ctzll(m);
   uint32_t base = 0;
-  if (lane == leader) base = add_relaxed(odd ? q.count[1] : q.count[0], static_cast<unsigned>(__builtin_popcountll(m)));
+  if (lane == leader) base = add_relaxed(count_of(q, k1), static_cast<unsigned>(__builtin_popcountll(m)));
   base = __shfl(base, leader);
   if (fresh) {
     const size_t slot = base + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << lane) - 1ull)));
-    unsigned long long *list = odd ? q.list[1] : q.list[0];
-    if (slot < q.cap) st_u64(&list[slot], (static_cast<unsigned long long>(si) << 32) | t);
+    if (slot < q.cap) st_u64(&list_of(q, k1)[slot], (static_cast<unsigned long long>(si) << 32) | mk.t);
   }
 }
 
 // Single-thread form for set-up kernels.
 __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t si, int k1, int X, int Y) {
-  const bool odd = (k1 & 1) != 0;
   const uint32_t t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
   const uint32_t bit = 1u << (t >> sc.nwords_log2);
-  uint32_t *mark = odd ? q.mark[1] : q.mark[0];
-  const uint32_t old = atomicOr(&mark[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit);
+  const uint32_t old = atomicOr(&mark_of(q, k1)[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit);
   if (old & bit) return;
-  const size_t slot = atomicAdd(odd ? q.count[1] : q.count[0], 1u);
-  if (slot < q.cap) (odd ? q.list[1] : q.list[0])[slot] = (static_cast<unsigned long long>(si) << 32) | t;
+  const size_t slot = atomicAdd(count_of(q, k1), 1u);
+  if (slot < q.cap) list_of(q, k1)[slot] = (static_cast<unsigned long long>(si) << 32) | t;
 }
 
-// A tile changed in sweep k: queue for sweep k + 1 its 3x3 tile
-// neighbourhood and the box of its far dependents grown by one tile (bx:
-// x0, y0, x1, y1 in lanes 0..3).
-__device__ inline void queue_next(const Args &a, const Scen &sc, uint32_t si, int k, int tx, int ty, uint32_t bx,
-                                  int lane) {
-  {
-    const int X = tx + lane % 3 - 1, Y = ty + lane / 3 - 1;
-    queue_tiles(a.q, sc, si, k + 1, lane < 9 && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y, lane);
-  }
+// The tiles a change of tile (tx,ty) in sweep k queues for sweep k + 1: its
+// 3x3 tile neighbourhood (lanes 0..8 of round 0) and the box of its far
+// dependents grown by one tile (bx: x0, y0, x1, y1 in lanes 0..3), box tile
+// q in lane 9 + q of round 0 and lane (q - 55) % 64 of round 1 + (q - 55) / 64.
+struct NextTiles {
+  int X0, Y0, w, n;  // grown box (n = 0: none)
+};
+
+__device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
+  NextTiles nt{0, 0, 1, 0};
   const uint32_t x0 = __shfl(bx, 0), y0 = __shfl(bx, 1), x1 = __shfl(bx, 2), y1 = __shfl(bx, 3);
   if (x0 <= x1 && y0 <= y1) {  // an empty box is x0 = y0 = 0xFFFFFFFF, x1 = y1 = 0
-    const int X0 = max(static_cast<int>(x0) - 1, 0), Y0 = max(static_cast<int>(y0) - 1, 0);
+    nt.X0 = max(static_cast<int>(x0) - 1, 0);
+    nt.Y0 = max(static_cast<int>(y0) - 1, 0);
     const int X1 = min(static_cast<int>(x1) + 1, sc.ntx - 1), Y1 = min(static_cast<int>(y1) + 1, sc.nty - 1);
-    const int w = X1 - X0 + 1, n = w * (Y1 - Y0 + 1);
-    for (int q0 = 0; q0 < n; q0 += kWave) {
-      const int q = q0 + lane;
-      queue_tiles(a.q, sc, si, k + 1, q < n, X0 + q % w, Y0 + q / w, lane);
-    }
+    nt.w = X1 - nt.X0 + 1;
+    nt.n = nt.w * (Y1 - nt.Y0 + 1);
   }
+  return nt;
+}
+
+__device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, int k, int tx, int ty,
+                                         const NextTiles &nt, int lane) {
+  if (lane < 9) {
+    const int X = tx + lane % 3 - 1, Y = ty + lane / 3 - 1;
+    return mark_issue(q, sc, k + 1, X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y);
+  }
+  const int b = lane - 9;
+  return mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+}
+
+// Rounds 1.. of the box; returns the number of tiles marked in all rounds.
+__device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_t si, int k, const NextTiles &nt,
+                                     int lane) {
+  for (int b0 = kWave - 9; b0 < nt.n; b0 += kWave) {
+    const int b = b0 + lane;
+    const Mark mk = mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+    mark_commit(q, si, k + 1, mk, lane);
+  }
+  return 9u + static_cast<uint32_t>(nt.n);
 }
 
 // One sweep over one tile by one wave: lane l owns cell (l % 8, l / 8) of
@@ -421,7 +480,14 @@ __device__ inline void queue_next(const Args &a, const Scen &sc, uint32_t si, in
 template <bool kCache, bool kLdsBits>
 __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned lds_off, int tile, int k, TileLds &s,
                           Counters &cnt) {
-  const Geom &g = sc.g;
+  // Descriptor fields in registers once (sc lives in global memory, so each
+  // use of a field after a store would otherwise reload it).
+  const Geom g = sc.g;
+  // one 16-byte load per coefficient record
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const PG_GLOBAL f4v *__restrict__ coef = G(reinterpret_cast<const f4v *>(sc.coef));
+  int *__restrict__ stamp = sc.stamp;
+  const uint32_t *__restrict__ bits = sc.bits;
   // (selects, not sc.st[k & 1]: a dynamic index would put sc in scratch)
   const uint2 *__restrict__ st_in = (k & 1) ? sc.st[1] : sc.st[0];
   uint2 *__restrict__ st_out = (k & 1) ? sc.st[0] : sc.st[1];
@@ -439,6 +505,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   constexpr int kPer = (kHaloCells + kWave - 1) / kWave;  // halo cells per lane
   uint2 w[kPer];
   int stp[kPer];
+  f4v cf[kPer];
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
     const int h = lane + it * kWave;
@@ -446,7 +513,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
     const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
     w[it] = inb ? ld_state(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
-    stp[it] = inb ? ld_i32(&sc.stamp[gi]) : kStampNever;
+    stp[it] = inb ? ld_i32(&stamp[gi]) : kStampNever;
+    cf[it] = coef[gi];
   }
   float rt[kPer];
   int rstp[kPer];
@@ -459,7 +527,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const size_t ri =
         gather ? static_cast<size_t>(ref_x<kCache>(w[it].y)) + static_cast<size_t>(ref_y<kCache>(w[it].y)) * g.W : 0;
     rt[it] = gather ? __uint_as_float(ld_u32(&st_in[ri].x)) : FLT_MAX;
-    rstp[it] = gather ? ld_i32(&sc.stamp[ri]) : kStampNever;
+    rstp[it] = gather ? ld_i32(&stamp[ri]) : kStampNever;
   }
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
@@ -468,6 +536,9 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       s.time[h] = __uint_as_float(w[it].x);
       s.ref[h] = w[it].y;
       s.rtime[h] = rt[it];
+      s.ca[h] = cf[it].x;  // Coef: a, b, r, null_fire
+      s.cb[h] = cf[it].y;
+      s.cr[h] = cf[it].w != 0.0f ? __builtin_inff() : cf[it].z;
       s.chg[h] = (stp[it] >= k - 1 || rstp[it] >= k - 1) ? 1 : 0;
     }
   }
@@ -487,19 +558,10 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   bool improved = false;
   bool far_dep = false;  // improved with a new reference two or more cells away
   uint32_t rtile = 0;    // that reference's tile
+  float best_t_out = 0.0f;
+  uint32_t best_ref_out = 0u;
   if (need) {
     ++cnt.relax;
-    float4 win[9];  // fire coefficients of the 3x3 neighbourhood
-#pragma unroll
-    for (int j = -1; j < 2; ++j) {
-#pragma unroll
-      for (int i = -1; i < 2; ++i) {
-        const int X = gx + i, Y = gy + j;
-        win[(j + 1) * 3 + (i + 1)] = (X >= 0 && Y >= 0 && X < g.W && Y < g.H)
-                                         ? sc.coef[static_cast<size_t>(X) + static_cast<size_t>(Y) * g.W]
-                                         : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-      }
-    }
     const int me = lx + ly * kHalo;
     const float own_t = s.time[me];
     float best_t = own_t;
@@ -516,8 +578,9 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         const int nl = (lx + i) + (ly + j) * kHalo;
         const float n_t = s.time[nl];
         if (!(n_t < g.max_time)) continue;
-        const float4 kf = win[(j + 1) * 3 + (i + 1)];
-        if (kf.w != 0.0f) continue;  // fire_is_null
+        const float kr = s.cr[nl];
+        if (__builtin_isinf(kr)) continue;  // fire_is_null (propag.h:16-80)
+        const float4 kf = make_float4(s.ca[nl], s.cb[nl], kr, 0.0f);
         const uint32_t ref = s.ref[nl];
         const int rfx = ref_x<kCache>(ref), rfy = ref_y<kCache>(ref);
         if (rfx == gx && rfy == gy) continue;
@@ -528,7 +591,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
           if (kCache)
             blocked = (ref & kClearBit) == 0u;
           else
-            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, sc.bits, lds_off, g.W, cnt.steps);
+            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, bits, lds_off, g.W, cnt.steps);
           const uint32_t cand_ref = blocked ? make_ref(nx, ny) : make_ref(rfx, rfy);
           const float cand_t = blocked ? nt : ct;
           if (cand_t < best_t) {
@@ -541,8 +604,24 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         }
       }
     }
-    const size_t gi = static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W;
     improved = best_t < own_t;  // update_point (propag.cu:192-200)
+    best_t_out = best_t;
+    best_ref_out = best_ref;
+  }
+  // A changed tile issues the marks of its next-sweep tiles first, so their
+  // round trip overlaps the line-of-sight walks below.
+  const bool changed = __ballot(improved) != 0ull;
+  NextTiles nt{0, 0, 1, 0};
+  Mark mk0{0u, 0u, 0u, false};
+  if (changed) {
+    nt = next_tiles(sc, bx);
+    mk0 = next_round0_issue(a.q, sc, k, tx, ty, nt, lane);
+  }
+  if (need) {
+    const int me = lx + ly * kHalo;
+    const size_t gi = static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W;
+    float best_t = best_t_out;
+    uint32_t best_ref = best_ref_out;
     if (improved) {
       const uint64_t tw = a.prof ? wall_clock64() : 0;
       const uint32_t old = s.ref[me];
@@ -552,7 +631,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         // keeps the reference keeps its flag.
         if (same_ref)
           best_ref = old;
-        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), sc.bits, lds_off, g.W,
+        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), bits, lds_off, g.W,
                                       cnt.steps))
           best_ref |= kClearBit;
       }
@@ -561,16 +640,17 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
       rtile = static_cast<uint32_t>(brx / kTile) + static_cast<uint32_t>(bry / kTile) * static_cast<uint32_t>(sc.ntx);
       st_state(&st_out[gi], pack_state(best_t, best_ref));
-      st_i32(&sc.stamp[gi], k);
+      st_i32(&stamp[gi], k);
       ++cnt.improve;
     } else {
-      st_state(&st_out[gi], pack_state(own_t, s.ref[me]));
+      st_state(&st_out[gi], pack_state(s.time[me], s.ref[me]));
     }
   }
   uint64_t tp2 = a.prof ? wall_clock64() : 0;
-  const bool changed = __ballot(improved) != 0ull;
   if (changed) {
-    queue_next(a, sc, si, k, tx, ty, bx, lane);
+    mark_commit(a.q, si, k + 1, mk0, lane);
+    const uint32_t marks = next_rest(a.q, sc, si, k, nt, lane);
+    if (lane == 0) cnt.marks += marks;
     // Register this tile as a dependent of each new far reference's tile,
     // one set of atomics per distinct reference tile.
     unsigned long long m = __ballot(far_dep);
@@ -615,15 +695,14 @@ template <bool kCache, bool kLdsBits>
 __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds &s,
                            Counters &cnt) {
   const unsigned lane = threadIdx.x & (kWave - 1);
-  const bool odd = (k & 1) != 0;
-  unsigned long long *list = odd ? a.q.list[1] : a.q.list[0];
-  uint32_t *mark = odd ? a.q.mark[1] : a.q.mark[0];
+  unsigned long long *list = list_of(a.q, k);
+  uint32_t *mark = mark_of(a.q, k);
   bool changed = false;
   for (size_t e0 = gwave; e0 < a.q.cap; e0 += static_cast<size_t>(nwaves) * kWave) {
     const size_t e = e0 + static_cast<size_t>(lane) * nwaves;
     unsigned long long v = kNoEntry;
     if (e < a.q.cap) v = ld_u64(&list[e]);
-    if (v != kNoEntry) st_u64(&list[e], kNoEntry);  // written again only two sweeps later
+    if (v != kNoEntry) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
     const unsigned long long m = __ballot(v != kNoEntry);
     for (unsigned long long r = m; r;) {
       const int l = __builtin_ctzll(r);
@@ -635,8 +714,7 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
       const Scen &sc = a.scen[si];
       if (lane == 0) {
         const uint32_t bit = 1u << (tile >> sc.nwords_log2);
-        (void)__hip_atomic_fetch_and(&mark[sc.wbase + (tile & ((1u << sc.nwords_log2) - 1u))], ~bit,
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        and_relaxed(&mark[sc.wbase + (tile & ((1u << sc.nwords_log2) - 1u))], ~bit);
       }
       if (static_cast<unsigned>(k - sc.k0) >= sc.max_iters) {
         if (lane == 0) st_u32(&a.out[si].capped, 1u);
@@ -757,9 +835,9 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
           st_relaxed(&c->book.v[1], changed ? 0u : 1u);
         }
         c->stats[6] += maxwork;  // only this lane ever writes it
-        // This sweep's list is consumed; sweep k + 1 appends to it again.
-        st_relaxed((k & 1) ? a.q.count[1] : a.q.count[0], 0u);
-        wait_stores();
+        // Sweep k + 2 appends to list (k + 3) & 3, last read in sweep k - 1;
+        // this store completes before this wave's next barrier arrival.
+        st_relaxed(count_of(a.q, k + 3), 0u);
         if (a.prof) {
           s_release = wall_clock64();
           if (row < a.prof_sweeps) {
@@ -792,10 +870,10 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
 
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
-  unsigned long long v[6] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                             cnt.wait_ticks};
+  unsigned long long v[8] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
+                             cnt.wait_ticks, 0ull, cnt.marks};
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
+  for (int q = 0; q < 8; ++q) {
     unsigned long long x = v[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(&a.ctl->stats[q], x);
@@ -826,7 +904,7 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
   }
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   for (int k = a.k_begin;; ++k) {
@@ -860,7 +938,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], cnt);
@@ -875,7 +953,7 @@ __global__ void sweep_book(Args a, int k) {
   Control *c = a.ctl;
   const unsigned ch = c->changes.v[k & 1];
   c->changes.v[(k + 1) & 1] = 0u;
-  *((k & 1) ? a.q.count[1] : a.q.count[0]) = 0u;
+  *count_of(a.q, k + 3) = 0u;
   const bool done = ch == 0u || k + 1 >= a.k_end;
   if (done) {
     c->book.v[4] = static_cast<unsigned>(k);
@@ -917,12 +995,12 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
     const float s = speed[i], e = ecc[i];
     double sn, cs;
     sincos_det(static_cast<double>(az[i]), sn, cs);
-    float4 k;
-    k.x = static_cast<float>(static_cast<double>(e) * cs);
-    k.y = static_cast<float>(static_cast<double>(e) * sn);
-    k.z = static_cast<float>(1.0 / (static_cast<double>(s) * (1.0 - static_cast<double>(e))));
-    k.w = s == 0.0f ? 1.0f : 0.0f;
-    const_cast<float4 *>(sc.coef)[i] = k;
+    Coef k;
+    k.a = static_cast<float>(static_cast<double>(e) * cs);
+    k.b = static_cast<float>(static_cast<double>(e) * sn);
+    k.r = static_cast<float>(1.0 / (static_cast<double>(s) * (1.0 - static_cast<double>(e))));
+    k.null_fire = s == 0.0f ? 1.0f : 0.0f;
+    const_cast<Coef *>(sc.coef)[i] = k;
     const float t = time[i];
     uint32_t ref = static_cast<uint32_t>(rx[i]) | (static_cast<uint32_t>(ry[i]) << 16);
     const bool burned = t < FLT_MAX;
@@ -1062,7 +1140,7 @@ size_t scen_buffer_bytes(size_t W, size_t H) {
   const size_t n = W * H;
   const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
   size_t off = 0;
-  off = align_up(off + n * sizeof(float4), 256);
+  off = align_up(off + n * sizeof(Coef), 256);
   off = align_up(off + n * sizeof(uint2), 256);
   off = align_up(off + n * sizeof(uint2), 256);
   off = align_up(off + n * sizeof(int), 256);
@@ -1075,7 +1153,7 @@ ScenBuffers scen_carve(void *base, size_t W, size_t H) {
   char *p = static_cast<char *>(base);
   size_t off = 0;
   ScenBuffers b;
-  b.coef = reinterpret_cast<float4 *>(p + off); off = align_up(off + n * sizeof(float4), 256);
+  b.coef = reinterpret_cast<Coef *>(p + off); off = align_up(off + n * sizeof(Coef), 256);
   b.st0 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.st1 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.stamp = reinterpret_cast<int *>(p + off); off = align_up(off + n * sizeof(int), 256);
@@ -1283,6 +1361,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     stats->improvements += counters[1];
     stats->dda_steps += counters[2];
     stats->tiles_visited += counters[3];
+    stats->tile_marks += counters[7];
     stats->relax_ms += rms;
     stats->work_ms += static_cast<double>(counters[4]) / clock_khz / groups_d;
     stats->wait_ms += static_cast<double>(counters[5]) / clock_khz / groups_d;
@@ -1327,8 +1406,8 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   const size_t o_scen = off; off = align_up(off + scen_buffer_bytes(W, H), 256);
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
   const size_t o_mark = off; off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
-  const size_t o_list = off; off = align_up(off + 2 * ntiles * sizeof(unsigned long long), 256);
-  const size_t o_count = off; off = align_up(off + 256, 256);
+  const size_t o_list = off; off = align_up(off + 4 * ntiles * sizeof(unsigned long long), 256);
+  const size_t o_count = off; off = align_up(off + 512, 256);
   const size_t o_desc = off; off = align_up(off + sizeof(Scen), 256);
   const size_t o_out = off; off = align_up(off + sizeof(ScenOut), 256);
   st = ensure(&pd->call, off);
@@ -1346,10 +1425,10 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   TileQueue q;
   q.mark[0] = reinterpret_cast<uint32_t *>(base + o_mark);
   q.mark[1] = q.mark[0] + nwords;
-  q.list[0] = reinterpret_cast<unsigned long long *>(base + o_list);
-  q.list[1] = q.list[0] + ntiles;
-  q.count[0] = reinterpret_cast<uint32_t *>(base + o_count);
-  q.count[1] = q.count[0] + 32;
+  for (int i = 0; i < 4; ++i) {
+    q.list[i] = reinterpret_cast<unsigned long long *>(base + o_list) + i * ntiles;
+    q.count[i] = reinterpret_cast<uint32_t *>(base + o_count) + 32 * i;
+  }
   q.cap = ntiles;
   Scen *d_scen = reinterpret_cast<Scen *>(base + o_desc);
   ScenOut *d_out = reinterpret_cast<ScenOut *>(base + o_out);
@@ -1366,11 +1445,11 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   } guard{{&ev0, &ev1}};
   PG_HIP(hipEventRecord(ev0, stream));
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
-  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((4 * ntiles + 255) / 256, cu * 8));
+  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * ntiles + 255) / 256, cu * 8));
   hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, q.mark[0], static_cast<size_t>(2 * nwords), 0u);
   hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t *>(q.list[0]), 4 * ntiles, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, q.count[0], static_cast<size_t>(64), 0u);
+                     reinterpret_cast<uint32_t *>(q.list[0]), 8 * ntiles, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count[0], static_cast<size_t>(128), 0u);
   PG_HIP(hipGetLastError());
   st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, stream);
   if (st != PG_OK) return st;

@@ -20,12 +20,18 @@ struct Geom {
   int north_up, square;
 };
 
+// Per-cell fire coefficients of time_from: A = e cos(az), B = e sin(az),
+// R = 1 / (speed (1 - e)); null_fire = 1 where speed == 0.
+struct alignas(16) Coef {
+  float a, b, r, null_fire;
+};
+
 // One scenario as the kernels read it (read-only during a launch).  Tile t
 // of the scenario is bit (t >> nwords_log2) of bitmap word
 // wbase + (t & (2^nwords_log2 - 1)).
 struct Scen {
   Geom g;
-  const float4 *coef;    // per cell: A, B, R, null flag
+  const Coef *coef;
   uint2 *st[2];          // double-buffered (time bits, reference word)
   int *stamp;            // last sweep in which the cell changed
   const uint32_t *bits;  // boundary bitset
@@ -46,17 +52,18 @@ struct ScenOut {
   unsigned pad;
 };
 
-// Per-sweep queues of tiles to visit, double-buffered by sweep parity: the
-// tiles marked for sweep k sit in list[k & 1][0 .. count[k & 1]) as
-// scenario << 32 | tile, each once (mark[k & 1] holds one bit per tile of
-// every scenario, in the scenario's word range).  Unused list slots hold
-// ~0.  The queues outlive a launch: a launch that stops at k_end leaves the
-// tiles of sweep k_end queued for the next one.
+// Per-sweep queues of tiles to visit.  The tiles marked for sweep k sit in
+// list[k & 3][0 .. count[k & 3]) as scenario << 32 | tile, each once:
+// mark[k & 1] holds one bit per tile of every scenario (in the scenario's
+// word range), set by the first marker and cleared by the reader.  Unused
+// list slots hold ~0.  Four lists rotate so that a count can be reset two
+// barriers before it is appended to again.  The queues outlive a launch: a
+// launch that stops at k_end leaves the tiles of sweep k_end queued.
 struct TileQueue {
   uint32_t *mark[2];
-  unsigned long long *list[2];
-  uint32_t *count[2];  // each on its own 128-byte line
-  size_t cap;          // list slots per parity (>= tiles of all scenarios)
+  unsigned long long *list[4];
+  uint32_t *count[4];  // each on its own 128-byte line
+  size_t cap;          // slots per list (>= tiles of all scenarios)
 };
 
 // Bitmap words a scenario of ntiles tiles uses (a power of two >= ntiles/32).
@@ -68,7 +75,7 @@ inline uint32_t need_words_log2(size_t ntiles) {
 
 // Device buffers of one scenario, carved by the caller.
 struct ScenBuffers {
-  float4 *coef;
+  Coef *coef;
   uint2 *st0, *st1;
   int *stamp;
   uint4 *box;

@@ -49,7 +49,8 @@ class Stats(C.Structure):
                 ("relax_ms", C.c_double), ("work_ms", C.c_double), ("wait_ms", C.c_double),
                 ("crit_ms", C.c_double),
                 ("workgroups", C.c_uint32), ("lds_bitset", C.c_uint32),
-                ("phase_ms", C.c_double * 4), ("barrier_ms", C.c_double * 4)]
+                ("phase_ms", C.c_double * 4), ("barrier_ms", C.c_double * 4),
+                ("tile_marks", C.c_uint64)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("phase_ms", "barrier_ms")}
