@@ -161,6 +161,79 @@ pg_status pg_propagate_host(const Settings *settings, const uint8_t *fuel,
                             uint32_t *boundaries, float *fire_out,
                             const pg_tuning *tuning, pg_stats *stats);
 
+/* ---------------------------------------------- batching and serving */
+
+/* Device-resident context: terrains stay on the device, scenarios on them
+ * are admitted while memory allows and advance together in shared
+ * persistent launches (several scenarios per launch, one grid barrier per
+ * sweep for all of them).  These are the handle variants of the device
+ * layer named in crates/propag/src/propag.rs:375-573; the reference has no
+ * batching form.  Every call is synchronous and thread-safe per context. */
+typedef struct pg_ctx pg_ctx;
+
+/* Weather of one scenario: moisture ratios (spatially constant, as the
+ * reference's rasters hold them in config-style runs) and the midflame wind
+ * (m/s; azimuth in radians, the direction the wind blows toward).  When the
+ * field pointers are not NULL they are host rasters of W*H values that
+ * replace the constant wind. */
+typedef struct pg_weather {
+  float d1hr, d10hr, d100hr, herb, wood;
+  float wind_speed, wind_azimuth;
+  const float *wind_speed_field, *wind_azimuth_field;
+} pg_weather;
+
+/* One ignited cell: column, row, ignition time (s). */
+typedef struct pg_ignition {
+  uint32_t x, y;
+  float time;
+} pg_ignition;
+
+/* Scenario states reported by pg_scenario_status. */
+enum { PG_SCENARIO_RUNNING = 0, PG_SCENARIO_DONE = 1, PG_SCENARIO_NOT_CONVERGED = 2 };
+
+/* Creates a context on the current device.  hbm_budget caps the bytes the
+ * context allocates for scenarios (0: 90 % of the free device memory). */
+pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **ctx);
+pg_status pg_ctx_destroy(pg_ctx *ctx);
+
+/* Copies a terrain to the device: fuel codes, slope (rise/run) and aspect
+ * (rad), W*H host arrays in the layout of geo.  Returns its id. */
+pg_status pg_terrain_add(pg_ctx *ctx, const GeoReference *geo, const uint8_t *fuel,
+                         const float *slope, const float *aspect, int *terrain_id);
+
+/* Bytes a scenario on a W x H terrain occupies while resident. */
+size_t pg_scenario_bytes(uint32_t width, uint32_t height);
+
+/* Admits a scenario: burn pass with its weather, initial state from the
+ * ignitions (duplicates keep the earliest time), boundaries and the queue
+ * of its first sweep, all on the device.  It joins the next pg_batch_run.
+ * Returns PG_ERR_BUSY when the memory budget has no room for it (retry once
+ * scenarios have been released) and PG_ERR_INVALID for an ignition outside
+ * the raster.  max_iterations 0 selects the default sweep cap. */
+pg_status pg_scenario_admit(pg_ctx *ctx, int terrain_id, const pg_weather *weather,
+                            const pg_ignition *ignitions, size_t n_ignitions,
+                            float max_time, uint32_t max_iterations, int *scenario_id);
+
+/* Runs every admitted scenario that has not finished, all in one persistent
+ * launch, for at most max_sweeps sweeps (0: until all finish).  Scenarios
+ * that finish get their post pass; *n_finished (may be NULL) receives how
+ * many finished in this call.  stats (may be NULL) receives the launch's
+ * counters. */
+pg_status pg_batch_run(pg_ctx *ctx, uint32_t max_sweeps, const pg_tuning *tuning,
+                       pg_stats *stats, uint32_t *n_finished);
+
+/* State of a scenario and the sweeps it has run. */
+pg_status pg_scenario_status(pg_ctx *ctx, int scenario_id, int *state, uint64_t *sweeps);
+
+/* Copies a finished scenario's results to host arrays (any may be NULL):
+ * time (W*H, FLT_MAX = not burned), refs_x/refs_y (W*H), boundaries
+ * (ceil(W*H/32) words) - the outputs of pg_propagate. */
+pg_status pg_scenario_result(pg_ctx *ctx, int scenario_id, float *time, uint16_t *refs_x,
+                             uint16_t *refs_y, uint32_t *boundaries);
+
+/* Frees a scenario's device memory (finished or not). */
+pg_status pg_scenario_release(pg_ctx *ctx, int scenario_id);
+
 #ifdef __cplusplus
 }
 #endif

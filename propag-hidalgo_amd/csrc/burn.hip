@@ -186,6 +186,34 @@ __global__ __launch_bounds__(256) void burn_scalar(size_t n, BurnArgs a, const D
   }
 }
 
+// A scenario on a resident terrain: fuel, slope and aspect per cell,
+// spatially constant moisture and either a constant wind or per-cell wind
+// rasters (null pointers select the constants).  Writes every cell: cells
+// without a catalog fuel get a null fire (speed 0), as the caller's
+// pre-zeroed outputs would hold after cuda_standard_simple_burn.
+struct WeatherArgs {
+  const uint8_t *fuel;
+  const float *slope, *aspect;
+  const float *ws_f, *waz_f;
+  float d1, d10, d100, herb, wood, ws, waz;
+  float *speed, *az, *ecc;
+};
+
+__global__ __launch_bounds__(256) void burn_weather(size_t n, WeatherArgs a, const DevCatalog *cat) {
+  __shared__ DevFuel s_cat[kNumFuels];
+  stage_catalog(s_cat, cat);
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float ws = a.ws_f ? a.ws_f[i] : a.ws;
+    const float waz = a.waz_f ? a.waz_f[i] : a.waz;
+    Fire f{0.0f, 0.0f, 0.0f};
+    (void)burn_cell(s_cat, a.fuel[i], a.d1, a.d10, a.d100, a.herb, a.wood, ws, waz, a.slope[i], a.aspect[i], f);
+    a.speed[i] = f.speed;
+    a.az[i] = f.az;
+    a.ecc[i] = f.ecc;
+  }
+}
+
 inline bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
 }  // namespace
@@ -212,6 +240,22 @@ pg_status launch_burn(const DeviceInfo *dev, size_t n, const uint8_t *fuel, cons
   else
     hipLaunchKernelGGL(burn_scalar, dim3(static_cast<unsigned>(blocks)), dim3(block), 0, stream, n, a,
                        dev->catalog);
+  PG_HIP(hipGetLastError());
+  return PG_OK;
+}
+
+pg_status launch_burn_weather(const DeviceInfo *dev, size_t n, const uint8_t *fuel, const float *slope,
+                              const float *aspect, const float moisture[5], float wind_speed, float wind_azimuth,
+                              const float *wind_speed_field, const float *wind_azimuth_field, float *speed,
+                              float *az, float *ecc, hipStream_t stream) {
+  if (n == 0) return PG_OK;
+  WeatherArgs a{fuel, slope, aspect, wind_speed_field, wind_azimuth_field,
+                moisture[0], moisture[1], moisture[2], moisture[3], moisture[4], wind_speed, wind_azimuth,
+                speed, az, ecc};
+  size_t blocks = (n + 255) / 256;
+  const size_t cap = static_cast<size_t>(dev->cu_count) * 16;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(burn_weather, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, n, a, dev->catalog);
   PG_HIP(hipGetLastError());
   return PG_OK;
 }

@@ -1,0 +1,511 @@
+// Device-resident batching context (pg_ctx_*, include/propag_hip.h).
+//
+// Terrains stay on the device.  A scenario on a terrain owns one device
+// block (its rasters, state, mark bitmaps, fire and result buffers) and a
+// slot in the context's descriptor array.  All unfinished scenarios advance
+// together in one persistent launch per pg_batch_run: they share the grid
+// barrier and the per-sweep tile queues, and each starts at the sweep that
+// was next when it was admitted, so a scenario admitted while others are
+// half-way joins the next launch (continuous batching at launch
+// granularity).  A scenario's result does not depend on what else runs:
+// its sweeps read only its own state.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "propag_dev.h"
+
+namespace pg {
+namespace {
+
+constexpr int kMaxSlots = 4096;
+
+struct Terrain {
+  GeoReference geo;
+  uint8_t *fuel = nullptr;
+  float *slope = nullptr, *aspect = nullptr;
+};
+
+struct Scenario {
+  bool used = false;
+  bool zombie = false;  // released while running: entries may still name the slot
+  int state = PG_SCENARIO_RUNNING;
+  bool posted = false;
+  int terrain = -1;
+  Scen sc{};
+  void *block = nullptr;
+  size_t bytes = 0;
+  size_t tiles = 0;
+  float *speed = nullptr, *az = nullptr, *ecc = nullptr, *time = nullptr;
+  uint16_t *rx = nullptr, *ry = nullptr;
+  uint32_t *bits = nullptr;
+  uint64_t sweeps = 0;
+};
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Layout {
+  size_t scen, mark, fire, time, refs, bits, total;
+};
+
+Layout layout_for(size_t W, size_t H) {
+  const size_t n = W * H;
+  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+  const size_t nwords = static_cast<size_t>(1) << need_words_log2(ntiles);
+  Layout L{};
+  size_t off = 0;
+  L.scen = off; off = align_up(off + scen_buffer_bytes(W, H), 256);
+  L.mark = off; off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
+  L.fire = off; off = align_up(off + 3 * n * sizeof(float), 256);
+  L.time = off; off = align_up(off + n * sizeof(float), 256);
+  L.refs = off; off = align_up(off + 2 * n * sizeof(uint16_t), 256);
+  L.bits = off; off = align_up(off + ((n + 31) / 32) * sizeof(uint32_t), 256);
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+}  // namespace pg
+
+struct pg_ctx {
+  pg::DeviceInfo *dev = nullptr;
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  size_t budget = 0, used = 0;
+  std::vector<pg::Terrain> terrains;
+  std::vector<pg::Scenario> scen;  // index = scenario id = slot
+  pg::Scen *d_scen = nullptr;
+  pg::ScenOut *d_out = nullptr;
+  pg::TileQueue q{};
+  void *q_block = nullptr;
+  pg_ignition *d_ign = nullptr;
+  size_t ign_cap = 0;
+  int k_next = 0;
+};
+
+namespace pg {
+namespace {
+
+// A dead descriptor: the kernels skip entries of a scenario with no sweeps
+// left before touching any of its buffers.
+Scen dead_descriptor() {
+  Scen s{};
+  s.max_iters = 0;
+  return s;
+}
+
+size_t live_tiles(const pg_ctx *c) {
+  size_t t = 0;
+  for (const Scenario &s : c->scen)
+    if (s.used || s.zombie) t += s.tiles;
+  return t;
+}
+
+// Grows the tile queues to `cap` slots per list, keeping the entries queued
+// for sweep k_next (the only list that holds entries between launches).
+pg_status grow_queue(pg_ctx *c, size_t cap) {
+  if (cap <= c->q.cap) return PG_OK;
+  cap = std::max(cap, 2 * c->q.cap);
+  const size_t list_bytes = align_up(cap * sizeof(unsigned long long), 256);
+  void *blk = nullptr;
+  hipError_t e = hipMalloc(&blk, 4 * list_bytes + 512);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(PG_ERR_OUT_OF_MEMORY, "tile queue allocation failed");
+  }
+  TileQueue nq{};
+  for (int i = 0; i < 4; ++i)
+    nq.list[i] = reinterpret_cast<unsigned long long *>(static_cast<char *>(blk) + i * list_bytes);
+  for (int i = 0; i < 4; ++i)
+    nq.count[i] = reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + 4 * list_bytes) + 32 * i;
+  nq.cap = cap;
+  pg_status st = scen_fill_u32(reinterpret_cast<uint32_t *>(blk), 4 * list_bytes / 4, 0xFFFFFFFFu, c->stream);
+  if (st != PG_OK) return st;
+  st = scen_fill_u32(nq.count[0], 128, 0u, c->stream);
+  if (st != PG_OK) return st;
+  if (c->q_block) {
+    PG_HIP(hipMemcpyAsync(nq.count[0], c->q.count[0], 512, hipMemcpyDeviceToDevice, c->stream));
+    const int p = c->k_next & 3;
+    uint32_t pending = 0;
+    PG_HIP(hipMemcpyAsync(&pending, c->q.count[p], sizeof(pending), hipMemcpyDeviceToHost, c->stream));
+    PG_HIP(hipStreamSynchronize(c->stream));
+    if (pending)
+      PG_HIP(hipMemcpyAsync(nq.list[p], c->q.list[p], pending * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                            c->stream));
+    PG_HIP(hipStreamSynchronize(c->stream));
+    PG_HIP(hipFree(c->q_block));
+  }
+  c->q_block = blk;
+  c->q = nq;
+  return PG_OK;
+}
+
+pg_status put_descriptor(pg_ctx *c, int slot, const Scen &sc) {
+  PG_HIP(hipMemcpyAsync(c->d_scen + slot, &sc, sizeof(Scen), hipMemcpyHostToDevice, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PG_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+size_t pg_scenario_bytes(uint32_t width, uint32_t height) { return layout_for(width, height).total; }
+
+pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **out) {
+  if (!out) return set_error(PG_ERR_INVALID, "pg_ctx_create: null output");
+  *out = nullptr;
+  DeviceInfo *dev = nullptr;
+  pg_status st = device_info(&dev);
+  if (st != PG_OK) return st;
+  auto *c = new pg_ctx();
+  c->dev = dev;
+  if (hbm_budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      delete c;
+      return set_error(PG_ERR_HIP, "hipMemGetInfo failed");
+    }
+    hbm_budget = free_b / 10 * 9;
+  }
+  c->budget = hbm_budget;
+  c->scen.reserve(kMaxSlots);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_scen), kMaxSlots * sizeof(Scen));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_out), kMaxSlots * sizeof(ScenOut));
+  if (e != hipSuccess) {
+    st = hip_error(e, "pg_ctx_create");
+    pg_ctx_destroy(c);
+    return st;
+  }
+  *out = c;
+  return PG_OK;
+}
+
+pg_status pg_ctx_destroy(pg_ctx *c) {
+  if (!c) return PG_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (Scenario &s : c->scen)
+    if (s.block) (void)hipFree(s.block);
+  for (Terrain &t : c->terrains) {
+    if (t.fuel) (void)hipFree(t.fuel);
+    if (t.slope) (void)hipFree(t.slope);
+    if (t.aspect) (void)hipFree(t.aspect);
+  }
+  if (c->q_block) (void)hipFree(c->q_block);
+  if (c->d_ign) (void)hipFree(c->d_ign);
+  if (c->d_scen) (void)hipFree(c->d_scen);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PG_OK;
+}
+
+pg_status pg_terrain_add(pg_ctx *c, const GeoReference *geo, const uint8_t *fuel, const float *slope,
+                         const float *aspect, int *terrain_id) {
+  if (!c || !geo || !fuel || !slope || !aspect || !terrain_id)
+    return set_error(PG_ERR_INVALID, "pg_terrain_add: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const size_t n = static_cast<size_t>(geo->width) * geo->height;
+  if (n == 0) return set_error(PG_ERR_INVALID, "terrain has no cells");
+  if (geo->width > 0x7fffu || geo->height > 0x7fffu)
+    return set_error(PG_ERR_INVALID, "context terrains need sides below 32768");
+  Terrain t;
+  t.geo = *geo;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&t.fuel), n);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&t.slope), n * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&t.aspect), n * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(t.fuel, fuel, n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t.slope, slope, n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t.aspect, aspect, n * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (t.fuel) (void)hipFree(t.fuel);
+    if (t.slope) (void)hipFree(t.slope);
+    if (t.aspect) (void)hipFree(t.aspect);
+    return hip_error(e, "pg_terrain_add");
+  }
+  c->terrains.push_back(t);
+  *terrain_id = static_cast<int>(c->terrains.size()) - 1;
+  return PG_OK;
+}
+
+pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, const pg_ignition *ign,
+                            size_t n_ign, float max_time, uint32_t max_iterations, int *scenario_id) {
+  if (!c || !w || (!ign && n_ign) || !scenario_id)
+    return set_error(PG_ERR_INVALID, "pg_scenario_admit: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (terrain_id < 0 || static_cast<size_t>(terrain_id) >= c->terrains.size())
+    return set_error(PG_ERR_INVALID, "unknown terrain id");
+  const Terrain &t = c->terrains[terrain_id];
+  Settings settings{};
+  settings.geo_ref = t.geo;
+  settings.max_time = max_time;
+  Scen sc;
+  pg_status st = scen_geometry(&settings, &sc);
+  if (st != PG_OK) return st;
+  if (max_iterations) sc.max_iters = max_iterations;
+  if (sc.max_iters > 0x3fffffffu) return set_error(PG_ERR_INVALID, "max_iterations too large");
+  const uint32_t W = t.geo.width, H = t.geo.height;
+  const size_t n = static_cast<size_t>(W) * H;
+  // One ignition per cell, the earliest time.
+  std::map<std::pair<uint32_t, uint32_t>, float> cells;
+  for (size_t i = 0; i < n_ign; ++i) {
+    if (ign[i].x >= W || ign[i].y >= H) return set_error(PG_ERR_INVALID, "ignition outside the raster");
+    if (!(ign[i].time < FLT_MAX)) return set_error(PG_ERR_INVALID, "ignition time must be below FLT_MAX");
+    auto key = std::make_pair(ign[i].y, ign[i].x);
+    auto it = cells.find(key);
+    if (it == cells.end() || ign[i].time < it->second) cells[key] = ign[i].time;
+  }
+  std::vector<pg_ignition> uniq;
+  uniq.reserve(cells.size());
+  for (const auto &kv : cells) uniq.push_back(pg_ignition{kv.first.second, kv.first.first, kv.second});
+
+  const Layout L = layout_for(W, H);
+  if (c->used + L.total > c->budget) return set_error(PG_ERR_BUSY, "memory budget full");
+  int slot = -1;
+  for (size_t i = 0; i < c->scen.size(); ++i)
+    if (!c->scen[i].used && !c->scen[i].zombie) {
+      slot = static_cast<int>(i);
+      break;
+    }
+  if (slot < 0) {
+    if (c->scen.size() >= static_cast<size_t>(kMaxSlots)) return set_error(PG_ERR_BUSY, "all scenario slots in use");
+    c->scen.emplace_back();
+    slot = static_cast<int>(c->scen.size()) - 1;
+  }
+  Scenario s;
+  hipError_t e = hipMalloc(&s.block, L.total);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(PG_ERR_BUSY, "device allocation failed");
+  }
+  char *b = static_cast<char *>(s.block);
+  const ScenBuffers sb = scen_carve(b + L.scen, W, H);
+  sc.coef = sb.coef;
+  sc.st[0] = sb.st0;
+  sc.st[1] = sb.st1;
+  sc.stamp = sb.stamp;
+  sc.box = sb.box;
+  const uint32_t nwords = 1u << sc.nwords_log2;
+  sc.mark[0] = reinterpret_cast<uint32_t *>(b + L.mark);
+  sc.mark[1] = sc.mark[0] + nwords;
+  s.speed = reinterpret_cast<float *>(b + L.fire);
+  s.az = s.speed + n;
+  s.ecc = s.az + n;
+  s.time = reinterpret_cast<float *>(b + L.time);
+  s.rx = reinterpret_cast<uint16_t *>(b + L.refs);
+  s.ry = s.rx + n;
+  s.bits = reinterpret_cast<uint32_t *>(b + L.bits);
+  sc.bits = s.bits;
+  sc.k0 = c->k_next;
+  s.sc = sc;
+  s.terrain = terrain_id;
+  s.bytes = L.total;
+  s.tiles = static_cast<size_t>(sc.ntx) * sc.nty;
+
+  auto fail = [&](pg_status code) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(s.block);
+    return code;
+  };
+  st = grow_queue(c, live_tiles(c) + s.tiles);
+  if (st != PG_OK) return fail(st);
+  // Per-cell wind, if given, goes through the time / refs buffers (8 bytes
+  // per cell) before the initial state overwrites them.
+  const float *wsf = nullptr, *wazf = nullptr;
+  if (w->wind_speed_field || w->wind_azimuth_field) {
+    if (!w->wind_speed_field || !w->wind_azimuth_field)
+      return fail(set_error(PG_ERR_INVALID, "wind fields come in pairs"));
+    e = hipMemcpyAsync(s.time, w->wind_speed_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.rx, w->wind_azimuth_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return fail(hip_error(e, "wind upload"));
+    wsf = s.time;
+    wazf = reinterpret_cast<const float *>(s.rx);
+  }
+  const float moist[5] = {w->d1hr, w->d10hr, w->d100hr, w->herb, w->wood};
+  st = launch_burn_weather(c->dev, n, t.fuel, t.slope, t.aspect, moist, w->wind_speed, w->wind_azimuth, wsf, wazf,
+                           s.speed, s.az, s.ecc, c->stream);
+  if (st != PG_OK) return fail(st);
+  if (uniq.size() > c->ign_cap) {
+    if (c->d_ign) (void)hipFree(c->d_ign);
+    c->d_ign = nullptr;
+    c->ign_cap = 0;
+    e = hipMalloc(reinterpret_cast<void **>(&c->d_ign), uniq.size() * sizeof(pg_ignition));
+    if (e != hipSuccess) return fail(hip_error(e, "ignition buffer"));
+    c->ign_cap = uniq.size();
+  }
+  if (!uniq.empty()) {
+    e = hipMemcpyAsync(c->d_ign, uniq.data(), uniq.size() * sizeof(pg_ignition), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return fail(hip_error(e, "ignition upload"));
+  }
+  st = scen_init_state(s.time, s.rx, s.ry, n, W, c->d_ign, uniq.size(), c->stream);
+  if (st != PG_OK) return fail(st);
+  st = scen_fill_u32(sc.mark[0], 2 * static_cast<size_t>(nwords), 0u, c->stream);
+  if (st != PG_OK) return fail(st);
+  st = scen_prepare(c->dev, sc, static_cast<uint32_t>(slot), c->d_out + slot, c->q, s.speed, s.az, s.ecc, s.time,
+                    s.rx, s.ry, s.bits, false, c->stream);
+  if (st != PG_OK) return fail(st);
+  st = put_descriptor(c, slot, sc);  // synchronises: the host vector above may go
+  if (st != PG_OK) return fail(st);
+  s.used = true;
+  c->scen[slot] = s;
+  c->used += s.bytes;
+  *scenario_id = slot;
+  return PG_OK;
+}
+
+pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, pg_stats *stats,
+                       uint32_t *n_finished) {
+  if (!c) return set_error(PG_ERR_INVALID, "pg_batch_run: null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_finished) *n_finished = 0;
+  if (stats) *stats = pg_stats{};
+  int nslots = 0;
+  long long k_end = c->k_next + 1;
+  bool any = false;
+  int only = -1, live = 0;
+  for (size_t i = 0; i < c->scen.size(); ++i) {
+    const Scenario &s = c->scen[i];
+    if (s.used || s.zombie) nslots = static_cast<int>(i) + 1;
+    if (s.used && s.state == PG_SCENARIO_RUNNING) {
+      any = true;
+      k_end = std::max<long long>(k_end, static_cast<long long>(s.sc.k0) + s.sc.max_iters + 1);
+      only = static_cast<int>(i);
+      ++live;
+    }
+  }
+  if (!any) return PG_OK;
+  if (max_sweeps) k_end = std::min<long long>(k_end, static_cast<long long>(c->k_next) + max_sweeps);
+  if (k_end > 0x7fff0000LL) return set_error(PG_ERR_INVALID, "sweep counter exhausted; recreate the context");
+
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  PG_HIP(hipEventCreate(&ev0));
+  PG_HIP(hipEventCreate(&ev1));
+  struct EventGuard {
+    hipEvent_t *e[2];
+    ~EventGuard() {
+      for (hipEvent_t *p : e)
+        if (*p) (void)hipEventDestroy(*p);
+    }
+  } guard{{&ev0, &ev1}};
+  PG_HIP(hipEventRecord(ev0, c->stream));
+  Launch L{};
+  L.scen = c->d_scen;
+  L.out = c->d_out;
+  L.nscen = nslots;
+  L.q = c->q;
+  L.k_begin = c->k_next;
+  L.k_end = static_cast<int>(k_end);
+  L.cache = 1;
+  if (live == 1 && nslots == 1) {
+    L.lds_bits_src = c->scen[only].bits;
+    const size_t n = static_cast<size_t>(c->scen[only].sc.g.W) * c->scen[only].sc.g.H;
+    L.lds_bits_words = (n + 31) / 32;
+  }
+  pg_stats local{};
+  LaunchResult res{};
+  pg_status st = run_sweeps(c->dev, L, tuning, &local, &res, c->stream);
+  if (st != PG_OK) return st;
+  std::vector<ScenOut> outs(nslots);
+  PG_HIP(hipMemcpyAsync(outs.data(), c->d_out, nslots * sizeof(ScenOut), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  uint32_t fin = 0;
+  for (int i = 0; i < nslots; ++i) {
+    Scenario &s = c->scen[i];
+    if (s.zombie) {  // its queued entries were consumed (and skipped) by this launch
+      s.zombie = false;
+      continue;
+    }
+    if (!s.used || s.state != PG_SCENARIO_RUNNING) continue;
+    const ScenOut &o = outs[i];
+    if (o.capped) {
+      s.state = PG_SCENARIO_NOT_CONVERGED;
+      s.sweeps = s.sc.max_iters;
+    } else if (o.last_change < res.last_sweep) {
+      s.state = PG_SCENARIO_DONE;
+      s.sweeps = static_cast<uint64_t>(o.last_change - s.sc.k0 + 2);
+    } else {
+      s.sweeps = static_cast<uint64_t>(res.last_sweep - s.sc.k0 + 1);
+      continue;
+    }
+    st = scen_finish(s.sc, scen_final_buf(s.sc, o), s.speed, s.time, s.rx, s.ry, c->stream);
+    if (st != PG_OK) return st;
+    s.posted = true;
+    ++fin;
+  }
+  c->k_next = res.last_sweep + 1;
+  PG_HIP(hipEventRecord(ev1, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  float ms = 0.0f;
+  PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  if (stats) {
+    *stats = local;
+    stats->iterations = static_cast<uint64_t>(res.last_sweep - L.k_begin + 1);
+    stats->elapsed_ms = ms;
+  }
+  if (n_finished) *n_finished = fin;
+  return PG_OK;
+}
+
+pg_status pg_scenario_status(pg_ctx *c, int id, int *state, uint64_t *sweeps) {
+  if (!c) return set_error(PG_ERR_INVALID, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (id < 0 || static_cast<size_t>(id) >= c->scen.size() || !c->scen[id].used)
+    return set_error(PG_ERR_INVALID, "unknown scenario id");
+  if (state) *state = c->scen[id].state;
+  if (sweeps) *sweeps = c->scen[id].sweeps;
+  return PG_OK;
+}
+
+pg_status pg_scenario_result(pg_ctx *c, int id, float *time, uint16_t *refs_x, uint16_t *refs_y,
+                             uint32_t *boundaries) {
+  if (!c) return set_error(PG_ERR_INVALID, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (id < 0 || static_cast<size_t>(id) >= c->scen.size() || !c->scen[id].used)
+    return set_error(PG_ERR_INVALID, "unknown scenario id");
+  const Scenario &s = c->scen[id];
+  if (s.state == PG_SCENARIO_RUNNING || !s.posted) return set_error(PG_ERR_BUSY, "scenario still running");
+  const size_t n = static_cast<size_t>(s.sc.g.W) * s.sc.g.H;
+  if (time) PG_HIP(hipMemcpyAsync(time, s.time, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  if (refs_x) PG_HIP(hipMemcpyAsync(refs_x, s.rx, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
+  if (refs_y) PG_HIP(hipMemcpyAsync(refs_y, s.ry, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
+  if (boundaries)
+    PG_HIP(hipMemcpyAsync(boundaries, s.bits, ((n + 31) / 32) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(hipStreamSynchronize(c->stream));
+  return PG_OK;
+}
+
+pg_status pg_scenario_release(pg_ctx *c, int id) {
+  if (!c) return set_error(PG_ERR_INVALID, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (id < 0 || static_cast<size_t>(id) >= c->scen.size() || !c->scen[id].used)
+    return set_error(PG_ERR_INVALID, "unknown scenario id");
+  Scenario &s = c->scen[id];
+  const bool running = s.state == PG_SCENARIO_RUNNING;
+  pg_status st = put_descriptor(c, id, dead_descriptor());
+  if (st != PG_OK) return st;
+  PG_HIP(hipFree(s.block));
+  c->used -= s.bytes;
+  const size_t tiles = s.tiles;
+  s = Scenario{};
+  // Tiles of a running scenario may still be queued for the next sweep:
+  // keep the slot (and its queue share) out of use until a launch has
+  // consumed them.
+  if (running) {
+    s.zombie = true;
+    s.tiles = tiles;
+  }
+  return PG_OK;
+}
+
+}  // extern "C"

@@ -72,6 +72,14 @@ pg_status launch_burn(const DeviceInfo *dev, size_t n, const uint8_t *fuel,
                       const float *waz, const float *slope, const float *aspect,
                       float *speed, float *az, float *ecc, hipStream_t stream);
 
+// Stage F for a scenario on a resident terrain (burn.hip): constant
+// moisture (d1, d10, d100, herb, wood), constant wind unless the per-cell
+// wind rasters are given.  Writes all n cells.
+pg_status launch_burn_weather(const DeviceInfo *dev, size_t n, const uint8_t *fuel, const float *slope,
+                              const float *aspect, const float moisture[5], float wind_speed, float wind_azimuth,
+                              const float *wind_speed_field, const float *wind_azimuth_field, float *speed,
+                              float *az, float *ecc, hipStream_t stream);
+
 // Stage P driver (propag.hip).
 pg_status run_propagate(const DeviceInfo *dev, const Settings *settings,
                         const float *speed, const float *az, const float *ecc,

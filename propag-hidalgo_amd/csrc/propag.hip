@@ -387,7 +387,7 @@ __device__ inline uint32_t *count_of(const TileQueue &q, int k) {
     default: return q.count[3];
   }
 }
-__device__ inline uint32_t *mark_of(const TileQueue &q, int k) { return (k & 1) ? q.mark[1] : q.mark[0]; }
+__device__ inline uint32_t *mark_of(const Scen &sc, int k) { return (k & 1) ? sc.mark[1] : sc.mark[0]; }
 
 // Queuing a tile for sweep k1 is split in two so that the first round trip
 // can overlap other work: mark_issue sets the tile's bit (lanes with
@@ -403,7 +403,7 @@ __device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, int k1, bo
   if (valid) {
     m.t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
     m.bit = 1u << (m.t >> sc.nwords_log2);
-    m.old = or_fetch(&mark_of(q, k1)[sc.wbase + (m.t & ((1u << sc.nwords_log2) - 1u))], m.bit);
+    m.old = or_fetch(&mark_of(sc, k1)[m.t & ((1u << sc.nwords_log2) - 1u)], m.bit);
   }
   return m;
 }
@@ -426,7 +426,7 @@ __device__ inline void mark_commit(const TileQueue &q, uint32_t si, int k1, cons
 __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t si, int k1, int X, int Y) {
   const uint32_t t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
   const uint32_t bit = 1u << (t >> sc.nwords_log2);
-  const uint32_t old = atomicOr(&mark_of(q, k1)[sc.wbase + (t & ((1u << sc.nwords_log2) - 1u))], bit);
+  const uint32_t old = atomicOr(&mark_of(sc, k1)[t & ((1u << sc.nwords_log2) - 1u)], bit);
   if (old & bit) return;
   const size_t slot = atomicAdd(count_of(q, k1), 1u);
   if (slot < q.cap) list_of(q, k1)[slot] = (static_cast<unsigned long long>(si) << 32) | t;
@@ -696,7 +696,6 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
                            Counters &cnt) {
   const unsigned lane = threadIdx.x & (kWave - 1);
   unsigned long long *list = list_of(a.q, k);
-  uint32_t *mark = mark_of(a.q, k);
   bool changed = false;
   for (size_t e0 = gwave; e0 < a.q.cap; e0 += static_cast<size_t>(nwaves) * kWave) {
     const size_t e = e0 + static_cast<size_t>(lane) * nwaves;
@@ -712,9 +711,10 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
       const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev));
       if (si >= static_cast<uint32_t>(a.nscen)) continue;
       const Scen &sc = a.scen[si];
+      if (sc.max_iters == 0u) continue;  // released scenario: its buffers are gone
       if (lane == 0) {
         const uint32_t bit = 1u << (tile >> sc.nwords_log2);
-        and_relaxed(&mark[sc.wbase + (tile & ((1u << sc.nwords_log2) - 1u))], ~bit);
+        and_relaxed(&mark_of(sc, k)[tile & ((1u << sc.nwords_log2) - 1u)], ~bit);
       }
       if (static_cast<unsigned>(k - sc.k0) >= sc.max_iters) {
         if (lane == 0) st_u32(&a.out[si].capped, 1u);
@@ -1090,6 +1090,29 @@ __global__ void post(Scen sc, int final_buf, const float *speed, float *time, ui
   }
 }
 
+// Initial state of a scenario from point ignitions (propag.rs:433-449 for
+// points): FLT_MAX / 0xFFFF everywhere, then each ignited cell its time and
+// itself as reference.  The host passes at most one ignition per cell.
+__global__ void init_state(float *time, uint16_t *rx, uint16_t *ry, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    time[i] = FLT_MAX;
+    rx[i] = 0xFFFFu;
+    ry[i] = 0xFFFFu;
+  }
+}
+
+__global__ void set_ignitions(float *time, uint16_t *rx, uint16_t *ry, const pg_ignition *ign, size_t nign,
+                              uint32_t W) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nign) return;
+  const pg_ignition g = ign[i];
+  const size_t c = static_cast<size_t>(g.x) + static_cast<size_t>(g.y) * W;
+  time[c] = g.time;
+  rx[c] = static_cast<uint16_t>(g.x);
+  ry[c] = static_cast<uint16_t>(g.y);
+}
+
 // ------------------------------------------------------------- workspaces
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -1190,7 +1213,7 @@ pg_status scen_geometry(const Settings *settings, Scen *sc) {
 
 pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
                        const float *speed, const float *az, const float *ecc, const float *time,
-                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries,
+                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries, bool validate,
                        hipStream_t stream) {
   const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
@@ -1210,11 +1233,34 @@ pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenO
                      refs_y);
   PG_HIP(hipGetLastError());
   unsigned e = 0;
-  PG_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, stream));
-  PG_HIP(hipStreamSynchronize(stream));
+  if (validate) {
+    PG_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, stream));
+    PG_HIP(hipStreamSynchronize(stream));
+  }
   hipLaunchKernelGGL(init_out, dim3(1), dim3(1), 0, stream, out, sc.k0);
   PG_HIP(hipGetLastError());
   if (e & 2u) return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
+  return PG_OK;
+}
+
+pg_status scen_init_state(float *time, uint16_t *refs_x, uint16_t *refs_y, size_t n, uint32_t width,
+                          const pg_ignition *ignitions, size_t n_ignitions, hipStream_t stream) {
+  size_t b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  hipLaunchKernelGGL(init_state, dim3(static_cast<unsigned>(b ? b : 1)), dim3(256), 0, stream, time, refs_x, refs_y,
+                     n);
+  if (n_ignitions)
+    hipLaunchKernelGGL(set_ignitions, dim3(static_cast<unsigned>((n_ignitions + 255) / 256)), dim3(256), 0, stream,
+                       time, refs_x, refs_y, ignitions, n_ignitions, width);
+  PG_HIP(hipGetLastError());
+  return PG_OK;
+}
+
+pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream) {
+  size_t b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  hipLaunchKernelGGL(fill_u32, dim3(static_cast<unsigned>(b ? b : 1)), dim3(256), 0, stream, p, n, v);
+  PG_HIP(hipGetLastError());
   return PG_OK;
 }
 
@@ -1420,11 +1466,10 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   sc.stamp = b.stamp;
   sc.box = b.box;
   sc.bits = boundaries;
-  sc.wbase = 0;
   sc.k0 = 0;
   TileQueue q;
-  q.mark[0] = reinterpret_cast<uint32_t *>(base + o_mark);
-  q.mark[1] = q.mark[0] + nwords;
+  sc.mark[0] = reinterpret_cast<uint32_t *>(base + o_mark);
+  sc.mark[1] = sc.mark[0] + nwords;
   for (int i = 0; i < 4; ++i) {
     q.list[i] = reinterpret_cast<unsigned long long *>(base + o_list) + i * ntiles;
     q.count[i] = reinterpret_cast<uint32_t *>(base + o_count) + 32 * i;
@@ -1446,12 +1491,12 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   PG_HIP(hipEventRecord(ev0, stream));
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
   const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * ntiles + 255) / 256, cu * 8));
-  hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, q.mark[0], static_cast<size_t>(2 * nwords), 0u);
+  hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, sc.mark[0], static_cast<size_t>(2 * nwords), 0u);
   hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
                      reinterpret_cast<uint32_t *>(q.list[0]), 8 * ntiles, 0xFFFFFFFFu);
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count[0], static_cast<size_t>(128), 0u);
   PG_HIP(hipGetLastError());
-  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, stream);
+  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, true, stream);
   if (st != PG_OK) return st;
   PG_HIP(hipMemcpyAsync(d_scen, &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
 

@@ -26,9 +26,9 @@ struct alignas(16) Coef {
   float a, b, r, null_fire;
 };
 
-// One scenario as the kernels read it (read-only during a launch).  Tile t
-// of the scenario is bit (t >> nwords_log2) of bitmap word
-// wbase + (t & (2^nwords_log2 - 1)).
+// One scenario as the kernels read it (read-only during a launch).  Its
+// tile t is bit (t >> nwords_log2) of word t & (2^nwords_log2 - 1) of the
+// scenario's mark bitmaps (one per sweep parity).
 struct Scen {
   Geom g;
   const Coef *coef;
@@ -36,8 +36,9 @@ struct Scen {
   int *stamp;            // last sweep in which the cell changed
   const uint32_t *bits;  // boundary bitset
   uint4 *box;            // per tile: tile box (x0, y0, x1, y1) of far dependents
+  uint32_t *mark[2];     // tiles queued for a sweep of parity 0 / 1
   int ntx, nty;
-  uint32_t wbase, nwords_log2;
+  uint32_t nwords_log2, pad0;
   int k0;                // sweep number of the scenario's first sweep
   unsigned max_iters;    // sweeps allowed from k0
   int cache;             // line-of-sight flags cached in reference words
@@ -53,14 +54,13 @@ struct ScenOut {
 };
 
 // Per-sweep queues of tiles to visit.  The tiles marked for sweep k sit in
-// list[k & 3][0 .. count[k & 3]) as scenario << 32 | tile, each once:
-// mark[k & 1] holds one bit per tile of every scenario (in the scenario's
-// word range), set by the first marker and cleared by the reader.  Unused
-// list slots hold ~0.  Four lists rotate so that a count can be reset two
-// barriers before it is appended to again.  The queues outlive a launch: a
-// launch that stops at k_end leaves the tiles of sweep k_end queued.
+// list[k & 3][0 .. count[k & 3]) as scenario << 32 | tile, each once (the
+// scenario's mark bitmap of parity k & 1 has the tile's bit set by the first
+// marker; the reader clears it).  Unused list slots hold ~0.  Four lists
+// rotate so that a count can be reset two barriers before it is appended
+// to again.  The queues outlive a launch: a launch that stops at k_end
+// leaves the tiles of sweep k_end queued.
 struct TileQueue {
-  uint32_t *mark[2];
   unsigned long long *list[4];
   uint32_t *count[4];  // each on its own 128-byte line
   size_t cap;          // slots per list (>= tiles of all scenarios)
@@ -90,13 +90,19 @@ pg_status scen_geometry(const Settings *settings, Scen *sc);
 // Boundaries, coefficients, initial state and first-sweep tiles of scenario
 // number si (start sweep sc.k0): writes `boundaries`, the scenario's
 // rasters, queues in q the tiles of sweep sc.k0 and resets *out.  The
-// scenario's bitmap words must be zero.  Returns PG_ERR_INVALID (after
-// synchronising the stream) if a burned cell has a reference outside the
-// raster.
+// scenario's mark bitmaps must be zero.  With `validate`, synchronises the
+// stream and returns PG_ERR_INVALID if a burned cell has a reference outside
+// the raster (such cells are dropped either way).
 pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
                        const float *speed, const float *az, const float *ecc, const float *time,
-                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries,
+                       const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries, bool validate,
                        hipStream_t stream);
+
+// Initial state from point ignitions (at most one per cell, device array).
+pg_status scen_init_state(float *time, uint16_t *refs_x, uint16_t *refs_y, size_t n, uint32_t width,
+                          const pg_ignition *ignitions, size_t n_ignitions, hipStream_t stream);
+
+pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream);
 
 // post_propagate for one scenario from state buffer `final_buf`.
 pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,

@@ -417,3 +417,132 @@ __all__ = [
     "PropagGdalError", "PropagValidationError", "PropagMPIError", "PropagNotConverged", "burn",
     "burn_device", "propagate_device", "initial_state", "Tuning", "Stats", "FLT_MAX", "NO_REF",
 ]
+
+
+# ------------------------------------------------------------ batching / serving
+
+RUNNING, DONE, NOT_CONVERGED = 0, 1, 2
+
+
+class Context:
+    """Device-resident batching context (the pg_ctx_* handle API).
+
+    Terrains (fuel, slope, aspect) are uploaded once; scenarios on them carry
+    their own weather, ignitions and max_time, are admitted while the memory
+    budget allows, and advance together in shared persistent launches
+    (``run``).  A scenario's result equals what ``propagate`` returns for the
+    same inputs, whatever else shares its launches."""
+
+    def __init__(self, hbm_budget: int = 0):
+        lib = _native.load()
+        h = C.c_void_p()
+        _raise_for(lib.pg_ctx_create(C.c_size_t(hbm_budget), C.byref(h)), "pg_ctx_create")
+        self._lib, self._h = lib, h
+        self._shapes = {}     # terrain id -> (height, width)
+        self._scen = {}       # scenario id -> terrain id
+
+    def close(self):
+        if self._h:
+            self._lib.pg_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_terrain(self, geo_ref: GeoReference, fuel, slope, aspect) -> int:
+        n = len(geo_ref)
+        fuel = np.ascontiguousarray(np.asarray(fuel, np.uint8).ravel())
+        slope = np.ascontiguousarray(np.asarray(slope, np.float32).ravel())
+        aspect = np.ascontiguousarray(np.asarray(aspect, np.float32).ravel())
+        for name, a in (("fuel", fuel), ("slope", slope), ("aspect", aspect)):
+            if a.size != n:
+                raise PropagValidationError(f"terrain '{name}' has {a.size} cells, expected {n}")
+        _validate_fuel(fuel)
+        g = geo_ref.to_c()
+        tid = C.c_int()
+        _raise_for(self._lib.pg_terrain_add(self._h, C.byref(g), _ptr(fuel), _ptr(slope), _ptr(aspect),
+                                            C.byref(tid)), "pg_terrain_add")
+        self._shapes[tid.value] = (geo_ref.height, geo_ref.width)
+        return tid.value
+
+    def admit(self, terrain_id: int, weather: dict, ignitions, max_time: float,
+              max_iterations: int = 0) -> int | None:
+        """Admits a scenario; returns its id, or None when the memory budget
+        has no room (PG_ERR_BUSY).  ``weather``: d1hr, d10hr, d100hr, herb,
+        wood, wind_speed (m/s), wind_azimuth (rad); wind_speed /
+        wind_azimuth may be rasters.  ``ignitions``: (x, y, t) cells."""
+        w = _native.Weather()
+        keep = []
+        for k in ("d1hr", "d10hr", "d100hr", "herb", "wood"):
+            setattr(w, k, float(weather[k]))
+        ws, waz = np.asarray(weather["wind_speed"], np.float32), np.asarray(weather["wind_azimuth"], np.float32)
+        if ws.size == 1 and waz.size == 1:
+            w.wind_speed, w.wind_azimuth = float(ws.item()), float(waz.item())
+        else:
+            h, wd = self._shapes[terrain_id]
+            n = h * wd
+
+            def field(v):
+                v = np.asarray(v, np.float32)
+                v = v.reshape(-1) if v.size == n else np.broadcast_to(v, (h, wd))
+                return np.ascontiguousarray(v, np.float32).ravel()
+
+            ws, waz = field(ws), field(waz)
+            keep += [ws, waz]
+            w.wind_speed_field, w.wind_azimuth_field = ws.ctypes.data, waz.ctypes.data
+        ign = (_native.Ignition * max(1, len(ignitions)))()
+        for i, (x, y, t) in enumerate(ignitions):
+            ign[i].x, ign[i].y, ign[i].time = int(x), int(y), float(t)
+        sid = C.c_int()
+        st = self._lib.pg_scenario_admit(self._h, int(terrain_id), C.byref(w), C.cast(ign, C.c_void_p),
+                                         C.c_size_t(len(ignitions)), C.c_float(max_time),
+                                         C.c_uint32(max_iterations), C.byref(sid))
+        if st == 7:
+            return None
+        _raise_for(st, "pg_scenario_admit")
+        self._scen[sid.value] = terrain_id
+        return sid.value
+
+    def run(self, max_sweeps: int = 0, tuning=None):
+        """One shared launch of all unfinished scenarios (max_sweeps 0: to
+        completion).  Returns (number finished, stats dict)."""
+        stats = Stats()
+        fin = C.c_uint32()
+        t = _tuning(tuning)
+        _raise_for(self._lib.pg_batch_run(self._h, C.c_uint32(max_sweeps), C.byref(t) if t else None,
+                                          C.byref(stats), C.byref(fin)), "pg_batch_run")
+        return fin.value, stats.as_dict()
+
+    def status(self, sid: int):
+        st, sw = C.c_int(), C.c_uint64()
+        _raise_for(self._lib.pg_scenario_status(self._h, int(sid), C.byref(st), C.byref(sw)), "pg_scenario_status")
+        return st.value, sw.value
+
+    def result(self, sid: int):
+        """(time, refs_x, refs_y, boundaries) of a finished scenario."""
+        h, w = self._shapes[self._scen[sid]]
+        n = h * w
+        time = np.empty(n, np.float32)
+        rx = np.empty(n, np.uint16)
+        ry = np.empty(n, np.uint16)
+        bits = np.empty((n + 31) // 32, np.uint32)
+        _raise_for(self._lib.pg_scenario_result(self._h, int(sid), _ptr(time), _ptr(rx), _ptr(ry), _ptr(bits)),
+                   "pg_scenario_result")
+        return time.reshape(h, w), rx, ry, bits
+
+    def release(self, sid: int):
+        _raise_for(self._lib.pg_scenario_release(self._h, int(sid)), "pg_scenario_release")
+        self._scen.pop(sid, None)
+
+    @staticmethod
+    def scenario_bytes(width: int, height: int) -> int:
+        return int(_native.load().pg_scenario_bytes(C.c_uint32(width), C.c_uint32(height)))

@@ -14,7 +14,9 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libpropag_hip.so")
 
 EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propagate",
-           "pg_propagate_host")
+           "pg_propagate_host", "pg_ctx_create", "pg_ctx_destroy", "pg_terrain_add",
+           "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
+           "pg_scenario_result", "pg_scenario_release")
 
 
 class GT(C.Structure):
@@ -33,6 +35,17 @@ class GeoReferenceC(C.Structure):
 
 class SettingsC(C.Structure):
     _fields_ = [("geo_ref", GeoReferenceC), ("max_time", C.c_float)]
+
+
+class Weather(C.Structure):
+    _fields_ = [("d1hr", C.c_float), ("d10hr", C.c_float), ("d100hr", C.c_float),
+                ("herb", C.c_float), ("wood", C.c_float), ("wind_speed", C.c_float),
+                ("wind_azimuth", C.c_float), ("wind_speed_field", C.c_void_p),
+                ("wind_azimuth_field", C.c_void_p)]
+
+
+class Ignition(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("time", C.c_float)]
 
 
 class Tuning(C.Structure):
@@ -93,6 +106,21 @@ def load(require_torch: bool = True):
     lib.pg_propagate_host.restype = C.c_int
     lib.pg_propagate_host.argtypes = [C.POINTER(SettingsC)] + [vp] * 15 + [C.POINTER(Tuning),
                                                                            C.POINTER(Stats)]
+    i32, u32 = C.c_int, C.c_uint32
+    for name, args in (("pg_ctx_create", [sz, C.POINTER(vp)]),
+                       ("pg_ctx_destroy", [vp]),
+                       ("pg_terrain_add", [vp, C.POINTER(GeoReferenceC), vp, vp, vp, C.POINTER(i32)]),
+                       ("pg_scenario_admit", [vp, i32, C.POINTER(Weather), vp, sz, C.c_float, u32,
+                                              C.POINTER(i32)]),
+                       ("pg_batch_run", [vp, u32, C.POINTER(Tuning), C.POINTER(Stats), C.POINTER(u32)]),
+                       ("pg_scenario_status", [vp, i32, C.POINTER(i32), C.POINTER(C.c_uint64)]),
+                       ("pg_scenario_result", [vp, i32, vp, vp, vp, vp]),
+                       ("pg_scenario_release", [vp, i32])):
+        f = getattr(lib, name)
+        f.restype = C.c_int
+        f.argtypes = args
+    lib.pg_scenario_bytes.restype = sz
+    lib.pg_scenario_bytes.argtypes = [u32, u32]
     _lib = lib
     return lib
 

@@ -1,0 +1,238 @@
+"""Batched and served forms (the pg_ctx_* handle API) on the GPU.
+
+The bar: every scenario's arrival times, references and boundary bits are
+bit-identical to the one-scenario path (pg_propagate_host, itself pinned to
+the oracle by test_gpu_propagate) on the same inputs, whatever else shares
+its launches - other scenarios, other raster sizes, scenarios admitted
+while it runs (continuous batching), launches cut after a few sweeps, and a
+running neighbour being released.  One batched scenario is also compared
+with the oracle directly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from py_propag import synth
+
+pytestmark = pytest.mark.gpu
+
+FLT_MAX = oracle.FLT_MAX
+
+
+def _geo(pp, w, h, cell=5.0):
+    return pp.GeoReference(w, h, "+proj=utm +zone=30", [0.0, cell, 0.0, 0.0, 0.0, cell])
+
+
+def _terrain(seed, w, h):
+    side = max(w, h)
+    land = synth.landscape(seed, side, hills=max(1, side * side // 50000), patches=max(4, side * side // 5000))
+    sl = lambda a: np.ascontiguousarray(np.asarray(a).reshape(side, side)[:h, :w].ravel())  # noqa: E731
+    return {"fuel": sl(land["fuel"]), "slope": sl(land["slope"]), "aspect": sl(land["aspect"])}
+
+
+def _weather(seed, w=None, h=None, field=False):
+    r = np.random.Generator(np.random.PCG64(seed))
+    d1 = np.float32(r.uniform(0.03, 0.12))
+    wx = {"d1hr": d1, "d10hr": np.float32(d1 + np.float32(0.01)), "d100hr": np.float32(d1 + np.float32(0.02)),
+          "herb": np.float32(r.uniform(0.3, 1.5)), "wood": np.float32(r.uniform(0.6, 1.5))}
+    if field:
+        yy, xx = np.meshgrid(np.arange(h), np.arange(w), indexing="ij")
+        wx["wind_speed"] = (6.0 + 2.0 * np.sin(xx / 37.0) * np.cos(yy / 23.0)).astype(np.float32)
+        wx["wind_azimuth"] = np.radians(200.0 + 30.0 * np.sin(yy / 41.0)).astype(np.float32)
+    else:
+        wx["wind_speed"] = np.float32(r.uniform(2.0, 12.0))
+        wx["wind_azimuth"] = np.float32(math.radians(r.uniform(0.0, 360.0)))
+    return wx
+
+
+def _ignitions(seed, w, h, k):
+    r = np.random.Generator(np.random.PCG64(seed + 17))
+    cells = set()
+    while len(cells) < k:
+        cells.add((int(r.integers(w // 6, w - w // 6)), int(r.integers(h // 6, h - h // 6))))
+    return [(x, y, float(i) * 60.0) for i, (x, y) in enumerate(sorted(cells))]
+
+
+def _full_terrain(land, wx, n):
+    t = {"fuel": land["fuel"], "slope": land["slope"], "aspect": land["aspect"]}
+    for k in ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed", "wind_azimuth"):
+        v = np.asarray(wx[k], np.float32)
+        t[k] = np.full(n, v.item(), np.float32) if v.size == 1 else np.ascontiguousarray(v.ravel())
+    return t
+
+
+def _alone(pp, land, w, h, wx, ign, max_time):
+    res = pp.propagate(pp.Settings(_geo(pp, w, h), max_time), _full_terrain(land, wx, w * h), ignitions=ign)
+    return res
+
+
+def _same(batched, alone):
+    t, rx, ry, bits = batched
+    assert np.array_equal(t.view(np.uint32), alone.arrival_times.view(np.uint32))
+    assert np.array_equal(rx, alone.refs_x) and np.array_equal(ry, alone.refs_y)
+    assert np.array_equal(bits, alone.boundary_change)
+
+
+def _case(seed, w, h, field=False):
+    return {"wx": _weather(seed, w, h, field), "ign": _ignitions(seed, w, h, 1 + seed % 3),
+            "max_time": float(1800.0 * (1 + seed % 7))}
+
+
+def test_batch_matches_alone_and_oracle(oracle_lib):
+    import py_propag as pp
+
+    w, h = 200, 150  # not a multiple of the tile edge
+    land = _terrain(11, w, h)
+    cases = [_case(s, w, h, field=(s == 3)) for s in range(6)]
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+        fin, stats = ctx.run()
+        assert fin == len(cases)
+        for sid, c in zip(ids, cases):
+            state, sweeps = ctx.status(sid)
+            assert state == pp.DONE
+            alone = _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"])
+            _same(ctx.result(sid), alone)
+            assert sweeps == alone.stats["iterations"]
+        # scenario 0's propagation against the oracle directly, on the same
+        # fire rasters (the GPU burn of the one-scenario path: the oracle's
+        # own f32 burn agrees with it only to 1e-5, which the propagation can
+        # amplify)
+        c = cases[0]
+        res = pp.propagate(pp.Settings(_geo(pp, w, h), c["max_time"]), _full_terrain(land, c["wx"], w * h),
+                           ignitions=c["ign"], return_fire=True)
+        fire = tuple(np.ascontiguousarray(f, np.float32).ravel() for f in res.fire)
+        time0, rx0, ry0 = synth.initial_state(w, h, c["ign"])
+        ref = oracle.propagate(oracle.grid(w, h, 5.0, 5.0, c["max_time"]), *fire, time0, rx0, ry0, nthreads=16)
+        t, rx, ry, bits = ctx.result(ids[0])
+        assert np.array_equal(t.ravel().view(np.uint32), ref["time"].view(np.uint32))
+        assert np.array_equal(rx, ref["refs_x"]) and np.array_equal(ry, ref["refs_y"])
+        assert np.array_equal(bits, ref["boundaries"])
+    print(f"batched 6 scenarios: {stats['iterations']} sweeps, {stats['tiles_visited']} tile visits")
+
+
+def test_continuous_batching_admits_between_short_launches():
+    import py_propag as pp
+
+    w, h = 160, 160
+    land = _terrain(5, w, h)
+    cases = [_case(s, w, h) for s in range(10, 17)]
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        pending = list(range(len(cases)))
+        ids = {}
+        launches = 0
+        while pending or any(ctx.status(s)[0] == pp.RUNNING for s in ids.values()):
+            for _ in range(2):  # two arrivals per launch
+                if pending:
+                    i = pending.pop(0)
+                    ids[i] = ctx.admit(tid, cases[i]["wx"], cases[i]["ign"], cases[i]["max_time"])
+            ctx.run(max_sweeps=9)
+            launches += 1
+            assert launches < 1000
+        for i, sid in ids.items():
+            assert ctx.status(sid)[0] == pp.DONE
+            c = cases[i]
+            _same(ctx.result(sid), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+    print(f"{len(cases)} scenarios over {launches} launches of at most 9 sweeps")
+
+
+def test_mixed_raster_sizes_share_a_launch():
+    import py_propag as pp
+
+    shapes = [(96, 72), (256, 200), (64, 300)]
+    with pp.Context() as ctx:
+        lands, tids, ids, cases = [], [], [], []
+        for j, (w, h) in enumerate(shapes):
+            land = _terrain(20 + j, w, h)
+            lands.append(land)
+            tids.append(ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"]))
+        for s in range(7):
+            j = s % len(shapes)
+            w, h = shapes[j]
+            c = _case(30 + s, w, h)
+            cases.append((j, c))
+            ids.append(ctx.admit(tids[j], c["wx"], c["ign"], c["max_time"]))
+        ctx.run()
+        for sid, (j, c) in zip(ids, cases):
+            w, h = shapes[j]
+            _same(ctx.result(sid), _alone(pp, lands[j], w, h, c["wx"], c["ign"], c["max_time"]))
+
+
+def test_release_of_a_running_scenario_and_slot_reuse():
+    import py_propag as pp
+
+    w, h = 128, 128
+    land = _terrain(8, w, h)
+    a, b, c = (_case(s, w, h) for s in (40, 41, 42))
+    a["max_time"] = b["max_time"] = 30000.0
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        ia = ctx.admit(tid, a["wx"], a["ign"], a["max_time"])
+        ib = ctx.admit(tid, b["wx"], b["ign"], b["max_time"])
+        ctx.run(max_sweeps=5)
+        assert ctx.status(ia)[0] == pp.RUNNING
+        ctx.release(ia)                      # its tiles are still queued
+        ic = ctx.admit(tid, c["wx"], c["ign"], c["max_time"])
+        assert ic != ia                      # the slot waits for the next launch
+        ctx.run()
+        _same(ctx.result(ib), _alone(pp, land, w, h, b["wx"], b["ign"], b["max_time"]))
+        _same(ctx.result(ic), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+        idd = ctx.admit(tid, a["wx"], a["ign"], a["max_time"])  # slot reusable now
+        ctx.run()
+        _same(ctx.result(idd), _alone(pp, land, w, h, a["wx"], a["ign"], a["max_time"]))
+
+
+def test_memory_budget_refuses_then_admits():
+    import py_propag as pp
+
+    w, h = 96, 96
+    land = _terrain(9, w, h)
+    c = _case(50, w, h)
+    budget = int(pp.Context.scenario_bytes(w, h) * 1.5)
+    with pp.Context(hbm_budget=budget) as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        s1 = ctx.admit(tid, c["wx"], c["ign"], c["max_time"])
+        assert s1 is not None
+        assert ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) is None
+        ctx.run()
+        ctx.release(s1)
+        s2 = ctx.admit(tid, c["wx"], c["ign"], c["max_time"])
+        assert s2 is not None
+        ctx.run()
+        _same(ctx.result(s2), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+
+
+def test_sweep_cap_reports_not_converged():
+    import py_propag as pp
+
+    w, h = 96, 96
+    land = _terrain(10, w, h)
+    c = _case(60, w, h)
+    c["max_time"] = 20000.0
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        s = ctx.admit(tid, c["wx"], c["ign"], c["max_time"], max_iterations=3)
+        ok = ctx.admit(tid, c["wx"], c["ign"], c["max_time"])
+        ctx.run()
+        assert ctx.status(s) == (pp.NOT_CONVERGED, 3)
+        assert ctx.status(ok)[0] == pp.DONE
+
+
+def test_single_scenario_1024_through_the_context():
+    import py_propag as pp
+
+    sc = synth.single_scenario(seed=1, side=1024)
+    t = sc["terrain"]
+    w = h = 1024
+    wx = {k: np.float32(t[k][0]) for k in ("d1hr", "d10hr", "d100hr", "herb", "wood")}
+    wx["wind_speed"], wx["wind_azimuth"] = t["wind_speed"], t["wind_azimuth"]
+    land = {"fuel": t["fuel"], "slope": t["slope"], "aspect": t["aspect"]}
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        sid = ctx.admit(tid, wx, sc["ignitions"], float(sc["max_time"]))
+        ctx.run()
+        _same(ctx.result(sid), _alone(pp, land, w, h, wx, sc["ignitions"], float(sc["max_time"])))
