@@ -1,25 +1,31 @@
 """Benchmark of the propagation hot path on one MI355X.
 
-Workload (one step = one scenario): the single-scenario landscape,
-1024x1024 cells of 5 m, 20 Gaussian hills, 200 Voronoi fuel patches,
-constant moisture, smooth wind 6+-2 m/s toward 225+-20 deg, ignition at
-(500, 500), max_time 36000 s.  The terrain rasters are resident in HBM
-before the timed region; a step resets the arrival-time state on the
-device, runs the burn pass (pg_burn) and the propagation (pg_propagate:
-boundaries, exact-Jacobi relaxation to convergence, post-pass).  Steps run
-one after another, one scenario per call (the plain drop-in form), so the
-value is scenarios/s of that form with p50/p99 step latency.
+Headline (value): scenarios/s of 1024x1024, 10-hour scenarios run batched.
+A step is one ensemble of --batch scenarios on the single-scenario
+landscape (1024x1024 cells of 5 m, 20 Gaussian hills, 200 Voronoi fuel
+patches, resident in HBM before the timed region), each with its own
+weather (1-h moisture U(0.03,0.12), 10-h/100-h +0.01/+0.02, herb U(0.3,1.5),
+wood U(0.6,1.5), wind U(2,12) m/s toward U(0,360) deg) and 1-4 point
+ignitions, max_time 36000 s: admission (burn pass, boundaries, initial
+state on the device) and one shared persistent launch to convergence
+through the batching context.
 
-Also reported on the same line:
+Beside it, on the same line:
+  * plain: the same scenarios one per call (pg_burn + pg_propagate, the
+    drop-in form), scenarios/s and p50/p99 latency per scenario, plus the
+    config-1 scenario itself (smooth wind field, ignition at (500,500));
+    the first scenarios of the ensemble are checked bit-identical between
+    the two forms;
   * roofline of the dominant kernel (the relaxation kernel), with
     algorithmic bytes = 24 B per cell relaxation + 8 B per committed
     improvement, timed with HIP events around that kernel on its stream;
   * the Rothermel burn pass alone on the 1024x1024 i.i.d. field (Mcells/s,
     49 B/cell);
-  * the CPU baseline: the oracle (a serial C transcription) on the same
-    scenario, one thread, and its burn pass on all host cores.
+  * the CPU baseline: the oracle (a serial C transcription) on the config-1
+    scenario, one thread, and its burn pass on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                       [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -44,11 +50,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def ensemble_cases(seed, count, W, H, max_time):
+    """Per-scenario weather and ignitions of the ensemble (seeded)."""
+    r = np.random.Generator(np.random.PCG64(seed))
+    cases = []
+    for _ in range(count):
+        d1 = np.float32(r.uniform(0.03, 0.12))
+        wx = {"d1hr": d1, "d10hr": np.float32(d1 + np.float32(0.01)),
+              "d100hr": np.float32(d1 + np.float32(0.02)), "herb": np.float32(r.uniform(0.3, 1.5)),
+              "wood": np.float32(r.uniform(0.6, 1.5)), "wind_speed": np.float32(r.uniform(2.0, 12.0)),
+              "wind_azimuth": np.float32(np.radians(r.uniform(0.0, 360.0)))}
+        k = int(r.integers(1, 5))
+        cells = set()
+        while len(cells) < k:
+            cells.add((int(r.integers(W // 8, W - W // 8)), int(r.integers(H // 8, H - H // 8))))
+        cases.append({"wx": wx, "ign": [(x, y, 0.0) for x, y in sorted(cells)], "max_time": max_time})
+    return cases
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=3, help="timed ensembles")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed ensembles")
+    ap.add_argument("--batch", type=int, default=128, help="scenarios per ensemble")
+    ap.add_argument("--plain", type=int, default=16, help="ensemble scenarios timed one per call")
     ap.add_argument("--mode", type=int, default=0, help="0 persistent kernel, 1 one launch per sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-phases", action="store_true",
@@ -68,6 +94,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    tuning = {"mode": args.mode, "profile": 1 if args.profile_phases else 0}
 
     # ---------------------------------------------------------- workload
     t0 = time.time()
@@ -77,57 +104,118 @@ def main():
     log(f"generated single_scenario_1024 in {time.time() - t0:.1f}s")
     g = pp.GeoReference(W, H, "+proj=utm +zone=30", [0.0, sc["dx"], 0.0, 0.0, 0.0, sc["dy"]])
     settings = pp.Settings(g, float(sc["max_time"]))
-    fuel = torch.from_numpy(sc["terrain"]["fuel"]).to(dev)
-    cols = [torch.from_numpy(np.ascontiguousarray(sc["terrain"][k], np.float32)).to(dev)
-            for k in FIELDS]
-    fire = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
-    t_init = torch.from_numpy(sc["time"]).to(dev)
-    rx_init = torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev)
-    ry_init = torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev)
-    t_buf = torch.empty_like(t_init)
-    rx_buf = torch.empty_like(rx_init)
-    ry_buf = torch.empty_like(ry_init)
-    bits = torch.zeros((n + 31) // 32, dtype=torch.int32, device=dev)
-    tuning = {"mode": args.mode, "profile": 1 if args.profile_phases else 0}
+    land = {k: sc["terrain"][k] for k in ("fuel", "slope", "aspect")}
+    cases = ensemble_cases(2024, args.batch, W, H, 36000.0)
 
-    def step():
-        t_buf.copy_(t_init)
-        rx_buf.copy_(rx_init)
-        ry_buf.copy_(ry_init)
+    fuel = torch.from_numpy(land["fuel"]).to(dev)
+    slope = torch.from_numpy(land["slope"]).to(dev)
+    aspect = torch.from_numpy(land["aspect"]).to(dev)
+    fire = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    t_buf = torch.empty(n, dtype=torch.float32, device=dev)
+    rx_buf = torch.empty(n, dtype=torch.int16, device=dev)
+    ry_buf = torch.empty(n, dtype=torch.int16, device=dev)
+    bits = torch.zeros((n + 31) // 32, dtype=torch.int32, device=dev)
+
+    def resident_case(c):
+        """Device columns and initial state of one ensemble scenario."""
+        cols = []
+        for k in FIELDS:
+            if k in ("slope", "aspect"):
+                cols.append(slope if k == "slope" else aspect)
+            else:
+                cols.append(torch.full((n,), float(c["wx"][k]), dtype=torch.float32, device=dev))
+        tm, rx, ry = synth.initial_state(W, H, c["ign"])
+        return cols, torch.from_numpy(tm).to(dev), torch.from_numpy(rx.view(np.int16)).to(dev), \
+            torch.from_numpy(ry.view(np.int16)).to(dev)
+
+    def plain_step(cols, t0_, rx0, ry0, st=settings):
+        t_buf.copy_(t0_)
+        rx_buf.copy_(rx0)
+        ry_buf.copy_(ry0)
         for f in fire:
             f.zero_()
         pp.burn_device(fuel, *cols, *fire, stream=stream)
-        return pp.propagate_device(settings, *fire, t_buf, rx_buf, ry_buf, bits, tuning=tuning,
-                                   stream=stream)
+        return pp.propagate_device(st, *fire, t_buf, rx_buf, ry_buf, bits, tuning=tuning, stream=stream)
+
+    # ------------------------------------------- plain: config-1 scenario
+    c1_cols = [torch.from_numpy(np.ascontiguousarray(sc["terrain"][k], np.float32)).to(dev) for k in FIELDS]
+    c1_init = (torch.from_numpy(sc["time"]).to(dev), torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev),
+               torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev))
+    for _ in range(2):
+        plain_step(c1_cols, *c1_init)
+    torch.cuda.synchronize()
+    c1_lat, c1_stats = [], []
+    for _ in range(10):
+        s0 = time.perf_counter()
+        c1_stats.append(plain_step(c1_cols, *c1_init))
+        c1_lat.append((time.perf_counter() - s0) * 1e3)
+    burned = int((t_buf < pp.FLT_MAX).sum().item())
+    st1 = c1_stats[-1]
+    relax_ms = float(np.mean([s["relax_ms"] for s in c1_stats]))
+    log(f"config-1 plain: {np.mean(c1_lat):.3f} ms/scenario, relax kernel {relax_ms:.3f} ms, "
+        f"{st1['iterations']} sweeps, {st1['relaxations']} relaxations, {st1['tiles_visited']} tile visits, "
+        f"{burned} burned; critical path {st1['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) "
+        f"{st1['phase_ms']}, barrier (start/span/gather/book) {st1['barrier_ms']}")
+
+    # ------------------------------------------ plain: ensemble scenarios
+    n_plain = min(args.plain, args.batch)
+    res_cases = [resident_case(c) for c in cases[:n_plain]]
+    plain_step(*res_cases[0])
+    torch.cuda.synchronize()
+    plain_lat, plain_out = [], []
+    for i, rc in enumerate(res_cases):
+        s0 = time.perf_counter()
+        plain_step(*rc)
+        plain_lat.append((time.perf_counter() - s0) * 1e3)
+        if i < 2:
+            plain_out.append(t_buf.cpu().numpy().copy())
+    plain_sps = 1e3 / float(np.mean(plain_lat))
+    log(f"ensemble scenarios one per call: {np.mean(plain_lat):.3f} ms/scenario ({plain_sps:.1f} scenarios/s), "
+        f"p50 {np.percentile(plain_lat, 50):.3f} p99 {np.percentile(plain_lat, 99):.3f} ms")
+    del res_cases
+
+    # ------------------------------------------------ batched: ensembles
+    ctx = pp.Context()
+    tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+
+    def ensemble():
+        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+        if any(i is None for i in ids):
+            raise RuntimeError("ensemble does not fit the memory budget")
+        fin, stats = ctx.run(tuning=tuning)
+        return ids, fin, stats
 
     for _ in range(args.warmup):
-        st = step()
+        ids, _, _ = ensemble()
+        for i in ids:
+            ctx.release(i)
     torch.cuda.synchronize()
-    lat, stats = [], []
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
+    ens_ms, ens_stats = [], []
     wall0 = time.perf_counter()
-    ev[0].record(stream)
     for k in range(args.steps):
         s0 = time.perf_counter()
-        stats.append(step())
-        ev[k + 1].record(stream)
-        lat.append((time.perf_counter() - s0) * 1e3)
-    torch.cuda.synchronize()
+        ids, fin, stats = ensemble()
+        ens_ms.append((time.perf_counter() - s0) * 1e3)
+        ens_stats.append(stats)
+        if fin != len(cases):
+            raise RuntimeError(f"{len(cases) - fin} scenarios did not finish")
+        if k == args.steps - 1:
+            parity = all(np.array_equal(ctx.result(ids[i])[0].ravel().view(np.uint32), plain_out[i].view(np.uint32))
+                         for i in range(len(plain_out)))
+        for i in ids:
+            ctx.release(i)
     wall = time.perf_counter() - wall0
-    burned = int((t_buf < pp.FLT_MAX).sum().item())
-    ms_per_step = wall * 1e3 / args.steps
-    relax_ms = np.array([s["relax_ms"] for s in stats])
-    relax = stats[-1]["relaxations"]
-    impr = stats[-1]["improvements"]
-    alg_bytes = 24.0 * relax + 8.0 * impr
-    achieved = alg_bytes / (relax_ms.mean() * 1e-3) / 1e9
-    log(f"propagation: {ms_per_step:.3f} ms/scenario, relax kernel {relax_ms.mean():.3f} ms, "
-        f"{stats[-1]['iterations']} sweeps, {relax} relaxations, {burned} burned; per workgroup "
-        f"work {stats[-1]['work_ms']:.3f} ms, barrier wait {stats[-1]['wait_ms']:.3f} ms, "
-        f"{stats[-1]['workgroups']} workgroups, LDS bitset {stats[-1]['lds_bitset']}, critical path "
-        f"{stats[-1]['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) {stats[-1]['phase_ms']}, "
-        f"barrier (start/span/gather/book) {stats[-1]['barrier_ms']}")
+    ctx.close()
+    value = args.batch * args.steps / wall
+    es = ens_stats[-1]
+    log(f"batched: {np.mean(ens_ms):.1f} ms per ensemble of {args.batch} ({value:.1f} scenarios/s, "
+        f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
+        f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits, {es['dda_steps']} walk steps; "
+        f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
+        f"barrier (start/span/gather/book) {es['barrier_ms']}")
+    ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
+    alg_bytes = 24.0 * es["relaxations"] + 8.0 * es["improvements"]
+    achieved = alg_bytes / (ens_relax_ms * 1e-3) / 1e9
 
     # ---------------------------------------------------- Rothermel field
     field = synth.rothermel_field(seed=0, side=1024)
@@ -172,49 +260,54 @@ def main():
         oracle.burn_f32(field, nthreads=cores)
         cpu_burn_s = time.perf_counter() - t1
         cpu = {"value": 1.0 / cpu_s, "unit": "scenarios/s", "cores": 1, "kind": "port",
-               "sample": "one full single_scenario_1024 (burn + serial exact-Jacobi propagation to "
+               "sample": "one full config-1 scenario (burn + serial exact-Jacobi propagation to "
                          f"convergence, {ref['sweeps']} sweeps) on 1 thread, {cpu_s:.2f} s",
                "rothermel_mcells_s": n / cpu_burn_s / 1e6, "rothermel_cores": cores}
         log(f"cpu baseline: {cpu_s:.2f} s per scenario (1 thread); burn {n / cpu_burn_s / 1e6:.1f} "
             f"Mcells/s on {cores} threads")
 
-    p50, p99 = (float(np.percentile(lat, q)) for q in (50, 99))
     out = {
         "metric": METRIC,
-        "value": args.steps / wall,
+        "value": value,
         "unit": "scenarios/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "single_scenario_1024", "cells": n, "cell_m": 5.0,
-                   "max_time_s": 36000.0, "form": "plain (one scenario per call)",
+        "config": {"workload": f"ensemble of {args.batch} 1024x1024 10-h scenarios on the config-1 landscape",
+                   "cells_per_scenario": n, "cell_m": 5.0, "max_time_s": 36000.0,
+                   "form": "batched (one shared persistent launch per ensemble)",
                    "schedule": "persistent" if args.mode == 0 else "one launch per sweep"},
-        "latency_ms": {"p50": p50, "p99": p99},
+        "latency_ms": {"ensemble_p50": float(np.percentile(ens_ms, 50)),
+                       "ensemble_p99": float(np.percentile(ens_ms, 99)),
+                       "plain_p50": float(np.percentile(plain_lat, 50)),
+                       "plain_p99": float(np.percentile(plain_lat, 99))},
+        "plain": {"value": plain_sps, "unit": "scenarios/s", "scenarios": n_plain,
+                  "speedup_batched": value / plain_sps, "parity_first_scenarios": bool(parity),
+                  "config1_ms": float(np.mean(c1_lat)), "config1_p50_ms": float(np.percentile(c1_lat, 50)),
+                  "config1_p99_ms": float(np.percentile(c1_lat, 99))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "propagate_persistent" if args.mode == 0 else "propagate_sweep",
-                     "kernel_ms": float(relax_ms.mean()), "algorithmic_bytes": alg_bytes},
+                     "kernel_ms": ens_relax_ms, "algorithmic_bytes": alg_bytes},
         "rothermel": {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,
                       "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS,
                       "workload": "rothermel_field_1024"},
         "cpu_baseline": cpu,
-        "propagation": {"sweeps": stats[-1]["iterations"], "relaxations": relax,
-                        "improvements": impr, "dda_steps": stats[-1]["dda_steps"],
-                        "tiles_visited": stats[-1]["tiles_visited"], "tile_marks": stats[-1]["tile_marks"],
-                        "burned_cells": burned,
-                        "elapsed_ms": float(np.mean([s["elapsed_ms"] for s in stats])),
-                        "work_ms": float(np.mean([s["work_ms"] for s in stats])),
-                        "wait_ms": float(np.mean([s["wait_ms"] for s in stats])),
-                        "workgroups": stats[-1]["workgroups"], "lds_bitset": stats[-1]["lds_bitset"],
-                        "crit_ms": float(np.mean([s["crit_ms"] for s in stats])),
-                        "phase_ms": stats[-1]["phase_ms"],
-                        "barrier_ms": stats[-1]["barrier_ms"]},
+        "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
+                                                       "tiles_visited", "tile_marks", "relax_ms", "elapsed_ms",
+                                                       "workgroups")},
+                        "config1": {"sweeps": st1["iterations"], "relaxations": st1["relaxations"],
+                                    "improvements": st1["improvements"], "dda_steps": st1["dda_steps"],
+                                    "tiles_visited": st1["tiles_visited"], "burned_cells": burned,
+                                    "relax_ms": relax_ms, "crit_ms": st1["crit_ms"],
+                                    "phase_ms": st1["phase_ms"], "barrier_ms": st1["barrier_ms"],
+                                    "lds_bitset": st1["lds_bitset"], "workgroups": st1["workgroups"]}},
     }
     print(json.dumps(out), flush=True)
 

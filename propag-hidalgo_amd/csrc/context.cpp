@@ -84,8 +84,12 @@ struct pg_ctx {
   pg::ScenOut *d_out = nullptr;
   pg::TileQueue q{};
   void *q_block = nullptr;
-  pg_ignition *d_ign = nullptr;
-  size_t ign_cap = 0;
+  // Ignition lists of admissions since the last synchronisation (the
+  // device copies read them asynchronously), one device buffer each.
+  std::vector<std::vector<pg_ignition>> host_ign;
+  std::vector<void *> dev_ign;
+  // Freed scenario blocks by size, reused before allocating.
+  std::multimap<size_t, void *> pool;
   int k_next = 0;
 };
 
@@ -120,24 +124,22 @@ pg_status grow_queue(pg_ctx *c, size_t cap) {
     return set_error(PG_ERR_OUT_OF_MEMORY, "tile queue allocation failed");
   }
   TileQueue nq{};
-  for (int i = 0; i < 4; ++i)
-    nq.list[i] = reinterpret_cast<unsigned long long *>(static_cast<char *>(blk) + i * list_bytes);
-  for (int i = 0; i < 4; ++i)
-    nq.count[i] = reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + 4 * list_bytes) + 32 * i;
-  nq.cap = cap;
+  nq.list = reinterpret_cast<unsigned long long *>(blk);
+  nq.count = reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + 4 * list_bytes);
+  nq.cap = list_bytes / sizeof(unsigned long long);
   pg_status st = scen_fill_u32(reinterpret_cast<uint32_t *>(blk), 4 * list_bytes / 4, 0xFFFFFFFFu, c->stream);
   if (st != PG_OK) return st;
-  st = scen_fill_u32(nq.count[0], 128, 0u, c->stream);
+  st = scen_fill_u32(nq.count, 128, 0u, c->stream);
   if (st != PG_OK) return st;
   if (c->q_block) {
-    PG_HIP(hipMemcpyAsync(nq.count[0], c->q.count[0], 512, hipMemcpyDeviceToDevice, c->stream));
+    PG_HIP(hipMemcpyAsync(nq.count, c->q.count, 512, hipMemcpyDeviceToDevice, c->stream));
     const int p = c->k_next & 3;
     uint32_t pending = 0;
-    PG_HIP(hipMemcpyAsync(&pending, c->q.count[p], sizeof(pending), hipMemcpyDeviceToHost, c->stream));
+    PG_HIP(hipMemcpyAsync(&pending, c->q.count + 32 * p, sizeof(pending), hipMemcpyDeviceToHost, c->stream));
     PG_HIP(hipStreamSynchronize(c->stream));
     if (pending)
-      PG_HIP(hipMemcpyAsync(nq.list[p], c->q.list[p], pending * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                            c->stream));
+      PG_HIP(hipMemcpyAsync(nq.list + p * nq.cap, c->q.list + p * c->q.cap, pending * sizeof(unsigned long long),
+                            hipMemcpyDeviceToDevice, c->stream));
     PG_HIP(hipStreamSynchronize(c->stream));
     PG_HIP(hipFree(c->q_block));
   }
@@ -146,10 +148,51 @@ pg_status grow_queue(pg_ctx *c, size_t cap) {
   return PG_OK;
 }
 
-pg_status put_descriptor(pg_ctx *c, int slot, const Scen &sc) {
-  PG_HIP(hipMemcpyAsync(c->d_scen + slot, &sc, sizeof(Scen), hipMemcpyHostToDevice, c->stream));
-  PG_HIP(hipStreamSynchronize(c->stream));
+// Copies a descriptor from a host object that stays valid until the next
+// synchronisation of the context's stream.
+pg_status put_descriptor(pg_ctx *c, int slot, const Scen *sc) {
+  PG_HIP(hipMemcpyAsync(c->d_scen + slot, sc, sizeof(Scen), hipMemcpyHostToDevice, c->stream));
   return PG_OK;
+}
+
+// After the context's stream has been synchronised: host staging is free.
+void drop_staging(pg_ctx *c) {
+  c->host_ign.clear();
+  for (void *p : c->dev_ign) (void)hipFree(p);
+  c->dev_ign.clear();
+}
+
+pg_status sync_ctx(pg_ctx *c) {
+  PG_HIP(hipStreamSynchronize(c->stream));
+  drop_staging(c);
+  return PG_OK;
+}
+
+void *take_block(pg_ctx *c, size_t bytes) {
+  auto it = c->pool.find(bytes);
+  if (it != c->pool.end()) {
+    void *p = it->second;
+    c->pool.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    // give pooled blocks back and retry once
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->pool) (void)hipFree(kv.second);
+    c->pool.clear();
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  return p;
+}
+
+const Scen *dead_descriptor_ptr() {
+  static const Scen dead = dead_descriptor();
+  return &dead;
 }
 
 }  // namespace
@@ -194,15 +237,16 @@ pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **out) {
 pg_status pg_ctx_destroy(pg_ctx *c) {
   if (!c) return PG_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_staging(c);
   for (Scenario &s : c->scen)
     if (s.block) (void)hipFree(s.block);
+  for (auto &kv : c->pool) (void)hipFree(kv.second);
   for (Terrain &t : c->terrains) {
     if (t.fuel) (void)hipFree(t.fuel);
     if (t.slope) (void)hipFree(t.slope);
     if (t.aspect) (void)hipFree(t.aspect);
   }
   if (c->q_block) (void)hipFree(c->q_block);
-  if (c->d_ign) (void)hipFree(c->d_ign);
   if (c->d_scen) (void)hipFree(c->d_scen);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -283,11 +327,9 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
     slot = static_cast<int>(c->scen.size()) - 1;
   }
   Scenario s;
-  hipError_t e = hipMalloc(&s.block, L.total);
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(PG_ERR_BUSY, "device allocation failed");
-  }
+  s.block = take_block(c, L.total);
+  if (!s.block) return set_error(PG_ERR_BUSY, "device allocation failed");
+  hipError_t e = hipSuccess;
   char *b = static_cast<char *>(s.block);
   const ScenBuffers sb = scen_carve(b + L.scen, W, H);
   sc.coef = sb.coef;
@@ -313,8 +355,8 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   s.tiles = static_cast<size_t>(sc.ntx) * sc.nty;
 
   auto fail = [&](pg_status code) {
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(s.block);
+    (void)sync_ctx(c);
+    c->pool.emplace(L.total, s.block);
     return code;
   };
   st = grow_queue(c, live_tiles(c) + s.tiles);
@@ -336,29 +378,36 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   st = launch_burn_weather(c->dev, n, t.fuel, t.slope, t.aspect, moist, w->wind_speed, w->wind_azimuth, wsf, wazf,
                            s.speed, s.az, s.ecc, c->stream);
   if (st != PG_OK) return fail(st);
-  if (uniq.size() > c->ign_cap) {
-    if (c->d_ign) (void)hipFree(c->d_ign);
-    c->d_ign = nullptr;
-    c->ign_cap = 0;
-    e = hipMalloc(reinterpret_cast<void **>(&c->d_ign), uniq.size() * sizeof(pg_ignition));
+  pg_ignition *d_ign = nullptr;
+  const size_t n_uniq = uniq.size();
+  if (n_uniq) {
+    e = hipMalloc(reinterpret_cast<void **>(&d_ign), n_uniq * sizeof(pg_ignition));
     if (e != hipSuccess) return fail(hip_error(e, "ignition buffer"));
-    c->ign_cap = uniq.size();
-  }
-  if (!uniq.empty()) {
-    e = hipMemcpyAsync(c->d_ign, uniq.data(), uniq.size() * sizeof(pg_ignition), hipMemcpyHostToDevice, c->stream);
+    c->dev_ign.push_back(d_ign);
+    c->host_ign.push_back(std::move(uniq));
+    const std::vector<pg_ignition> &hv = c->host_ign.back();
+    e = hipMemcpyAsync(d_ign, hv.data(), n_uniq * sizeof(pg_ignition), hipMemcpyHostToDevice, c->stream);
     if (e != hipSuccess) return fail(hip_error(e, "ignition upload"));
   }
-  st = scen_init_state(s.time, s.rx, s.ry, n, W, c->d_ign, uniq.size(), c->stream);
+  st = scen_init_state(s.time, s.rx, s.ry, n, W, d_ign, n_uniq, c->stream);
   if (st != PG_OK) return fail(st);
   st = scen_fill_u32(sc.mark[0], 2 * static_cast<size_t>(nwords), 0u, c->stream);
   if (st != PG_OK) return fail(st);
   st = scen_prepare(c->dev, sc, static_cast<uint32_t>(slot), c->d_out + slot, c->q, s.speed, s.az, s.ecc, s.time,
                     s.rx, s.ry, s.bits, false, c->stream);
   if (st != PG_OK) return fail(st);
-  st = put_descriptor(c, slot, sc);  // synchronises: the host vector above may go
-  if (st != PG_OK) return fail(st);
   s.used = true;
-  c->scen[slot] = s;
+  c->scen[slot] = s;  // stable storage (reserved vector) for the async descriptor copy
+  st = put_descriptor(c, slot, &c->scen[slot].sc);
+  if (st != PG_OK) {
+    c->scen[slot] = Scenario{};
+    return fail(st);
+  }
+  // Caller-owned wind rasters are read by the copies above: finish them.
+  if (wsf) {
+    st = sync_ctx(c);
+    if (st != PG_OK) return st;
+  }
   c->used += s.bytes;
   *scenario_id = slot;
   return PG_OK;
@@ -445,7 +494,8 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   }
   c->k_next = res.last_sweep + 1;
   PG_HIP(hipEventRecord(ev1, c->stream));
-  PG_HIP(hipStreamSynchronize(c->stream));
+  st = sync_ctx(c);
+  if (st != PG_OK) return st;
   float ms = 0.0f;
   PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
   if (stats) {
@@ -481,8 +531,7 @@ pg_status pg_scenario_result(pg_ctx *c, int id, float *time, uint16_t *refs_x, u
   if (refs_y) PG_HIP(hipMemcpyAsync(refs_y, s.ry, n * sizeof(uint16_t), hipMemcpyDeviceToHost, c->stream));
   if (boundaries)
     PG_HIP(hipMemcpyAsync(boundaries, s.bits, ((n + 31) / 32) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  PG_HIP(hipStreamSynchronize(c->stream));
-  return PG_OK;
+  return sync_ctx(c);
 }
 
 pg_status pg_scenario_release(pg_ctx *c, int id) {
@@ -492,9 +541,11 @@ pg_status pg_scenario_release(pg_ctx *c, int id) {
     return set_error(PG_ERR_INVALID, "unknown scenario id");
   Scenario &s = c->scen[id];
   const bool running = s.state == PG_SCENARIO_RUNNING;
-  pg_status st = put_descriptor(c, id, dead_descriptor());
+  pg_status st = put_descriptor(c, id, dead_descriptor_ptr());
   if (st != PG_OK) return st;
-  PG_HIP(hipFree(s.block));
+  // Work on the context's stream is ordered: the block can be handed to a
+  // later admission without waiting.
+  c->pool.emplace(s.bytes, s.block);
   c->used -= s.bytes;
   const size_t tiles = s.tiles;
   s = Scenario{};

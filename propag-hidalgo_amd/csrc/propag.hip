@@ -56,7 +56,10 @@ namespace {
 
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 constexpr int kStampNever = INT_MIN / 2;
-constexpr int kWalkBatch = 8;           // DDA cells tested per batch
+// DDA cells tested per batch: the bit loads of a batch are in flight
+// together; global (L2) loads need more of them than LDS loads.
+template <bool kLdsBits>
+constexpr int walk_batch() { return kLdsBits ? 8 : 16; }
 constexpr int kWave = 64;               // lanes per wavefront
 
 // ------------------------------------------------------------ control block
@@ -263,6 +266,7 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *gbi
   float tmx = dlx, tmy = dly;
   int idx = fx + fy * W;
   for (;;) {
+    constexpr int kWalkBatch = walk_batch<kLdsBits>();
     uint32_t id[kWalkBatch];
     int live[kWalkBatch];
 #pragma unroll
@@ -285,6 +289,7 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *gbi
       steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
       return true;
     }
+    static_assert(kWalkBatch <= 32, "hits is a 32-bit mask");
     if ((ru | rv) < 0) {
 #pragma unroll
       for (int b = 0; b < kWalkBatch; ++b) steps += live[b] >= 0 ? 1u : 0u;
@@ -371,22 +376,12 @@ __device__ inline void wave_sync() {
 
 constexpr unsigned long long kNoEntry = ~0ull;
 
+// (base + stride, not an array of pointers: a dynamic index into the
+// kernel argument would move the whole argument block to scratch)
 __device__ inline unsigned long long *list_of(const TileQueue &q, int k) {
-  switch (k & 3) {
-    case 0: return q.list[0];
-    case 1: return q.list[1];
-    case 2: return q.list[2];
-    default: return q.list[3];
-  }
+  return q.list + static_cast<size_t>(k & 3) * q.cap;
 }
-__device__ inline uint32_t *count_of(const TileQueue &q, int k) {
-  switch (k & 3) {
-    case 0: return q.count[0];
-    case 1: return q.count[1];
-    case 2: return q.count[2];
-    default: return q.count[3];
-  }
-}
+__device__ inline uint32_t *count_of(const TileQueue &q, int k) { return q.count + 32 * (k & 3); }
 __device__ inline uint32_t *mark_of(const Scen &sc, int k) { return (k & 1) ? sc.mark[1] : sc.mark[0]; }
 
 // Queuing a tile for sweep k1 is split in two so that the first round trip
@@ -1470,10 +1465,8 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   TileQueue q;
   sc.mark[0] = reinterpret_cast<uint32_t *>(base + o_mark);
   sc.mark[1] = sc.mark[0] + nwords;
-  for (int i = 0; i < 4; ++i) {
-    q.list[i] = reinterpret_cast<unsigned long long *>(base + o_list) + i * ntiles;
-    q.count[i] = reinterpret_cast<uint32_t *>(base + o_count) + 32 * i;
-  }
+  q.list = reinterpret_cast<unsigned long long *>(base + o_list);
+  q.count = reinterpret_cast<uint32_t *>(base + o_count);
   q.cap = ntiles;
   Scen *d_scen = reinterpret_cast<Scen *>(base + o_desc);
   ScenOut *d_out = reinterpret_cast<ScenOut *>(base + o_out);
@@ -1493,8 +1486,8 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * ntiles + 255) / 256, cu * 8));
   hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, sc.mark[0], static_cast<size_t>(2 * nwords), 0u);
   hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t *>(q.list[0]), 8 * ntiles, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count[0], static_cast<size_t>(128), 0u);
+                     reinterpret_cast<uint32_t *>(q.list), 8 * ntiles, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count, static_cast<size_t>(128), 0u);
   PG_HIP(hipGetLastError());
   st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, true, stream);
   if (st != PG_OK) return st;

@@ -61,9 +61,9 @@ struct ScenOut {
 // to again.  The queues outlive a launch: a launch that stops at k_end
 // leaves the tiles of sweep k_end queued.
 struct TileQueue {
-  unsigned long long *list[4];
-  uint32_t *count[4];  // each on its own 128-byte line
-  size_t cap;          // slots per list (>= tiles of all scenarios)
+  unsigned long long *list;  // list p at list + p * cap
+  uint32_t *count;           // count p at count + 32 * p (one 128-byte line each)
+  size_t cap;                // slots per list (>= tiles of all scenarios)
 };
 
 // Bitmap words a scenario of ntiles tiles uses (a power of two >= ntiles/32).
