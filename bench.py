@@ -212,6 +212,7 @@ def main():
         f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
         f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits, {es['dda_steps']} walk steps; "
         f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
+        f"phase totals over waves {es['phase_total_ms']}, "
         f"barrier (start/span/gather/book) {es['barrier_ms']}")
     ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
     alg_bytes = 24.0 * es["relaxations"] + 8.0 * es["improvements"]

@@ -90,6 +90,8 @@ typedef struct pg_stats {
                                arrival until the master had seen all arrivals,
                                and the master's bookkeeping                   */
   uint64_t tile_marks;      /* tiles queued for a next sweep (with repeats)   */
+  double   phase_total_ms[4]; /* with tuning.profile: the phases of phase_ms
+                                 summed over all waves and sweeps (wave-ms)  */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

@@ -73,8 +73,8 @@ struct Control {
   Line gen;          // [0] barrier release: sweep + 1, bit 31 = finished
   Line book;         // [1] done, [3] error bits, [4] last sweep run
   Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
-  unsigned long long stats[8];  // relaxations, improvements, dda_steps, tiles, work, wait,
-                                // critical-path ticks
+  unsigned long long stats[12];  // relaxations, improvements, dda_steps, tiles, work, wait,
+                                 // critical-path ticks, marks, phase ticks summed over waves [4]
 };
 
 // Every pointer the kernels use points to device memory; pointers read from
@@ -344,6 +344,7 @@ struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
 };
 static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
 
+
 // Per-workgroup arrival slot of the grid barrier, one 128-byte line each.
 struct alignas(128) Slot {
   unsigned long long arrive;
@@ -364,6 +365,7 @@ struct Counters {
   uint32_t relax, improve, steps, tiles, marks;
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
+  uint64_t ph_total[4];             // profiling: this wave's phase ticks, all sweeps
 };
 
 __device__ inline void wave_sync() {
@@ -372,6 +374,46 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Deferred line-of-sight walks.  When the boundary bitset is read from L2,
+// a tile's commit walks would keep only its few improving lanes busy for
+// the length of the longest walk; instead each improving cell with a new
+// reference leaves its state here and the wave runs up to kWalkQ walks at
+// once (one per lane), writing each cell's state word when its flag is
+// known - still within the sweep, so the next sweep sees the same state.
+struct WalkReq {
+  uint32_t si;    // scenario
+  uint32_t cell;  // x | y << 16
+  uint32_t ref;   // reference word without the flag
+  float time;
+};
+constexpr int kWalkQ = kWave;
+
+template <bool kCache, bool kLdsBits>
+constexpr bool defer_walks() { return kCache && !kLdsBits; }
+
+struct WalkQueue {
+  WalkReq *q;  // this wave's kWalkQ slots (LDS)
+  unsigned n;  // queued (wave-uniform)
+};
+
+// Runs the queued walks, one per lane, and writes those cells' states.
+__device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
+  if (wq.n == 0) return;
+  wave_sync();
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  if (lane < wq.n) {
+    const WalkReq e = wq.q[lane];
+    const Scen &sc = a.scen[e.si];
+    const int x = static_cast<int>(e.cell & 0xffffu), y = static_cast<int>(e.cell >> 16);
+    uint32_t ref = e.ref;
+    if (line_clear<false>(x, y, ref_x<true>(ref), ref_y<true>(ref), sc.bits, 0, sc.g.W, cnt.steps)) ref |= kClearBit;
+    uint2 *st_out = (k & 1) ? sc.st[0] : sc.st[1];
+    st_state(&st_out[static_cast<size_t>(x) + static_cast<size_t>(y) * sc.g.W], pack_state(e.time, ref));
+  }
+  wq.n = 0;
+  wave_sync();
 }
 
 constexpr unsigned long long kNoEntry = ~0ull;
@@ -474,7 +516,8 @@ __device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_
 // committed an improvement.
 template <bool kCache, bool kLdsBits>
 __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned lds_off, int tile, int k, TileLds &s,
-                          Counters &cnt) {
+                          WalkQueue &wq, Counters &cnt) {
+  constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
   // Descriptor fields in registers once (sc lives in global memory, so each
   // use of a field after a store would otherwise reload it).
   const Geom g = sc.g;
@@ -551,6 +594,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       for (int i = -1; i < 2; ++i) need = need || s.chg[(lx + i) + (ly + j) * kHalo];
   }
   bool improved = false;
+  bool deferred = false;  // improved with a new reference whose walk is queued
   bool far_dep = false;  // improved with a new reference two or more cells away
   uint32_t rtile = 0;    // that reference's tile
   float best_t_out = 0.0f;
@@ -626,6 +670,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         // keeps the reference keeps its flag.
         if (same_ref)
           best_ref = old;
+        else if (kDefer)
+          deferred = true;
         else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), bits, lds_off, g.W,
                                       cnt.steps))
           best_ref |= kClearBit;
@@ -634,11 +680,28 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       const int brx = ref_x<kCache>(best_ref), bry = ref_y<kCache>(best_ref);
       far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
       rtile = static_cast<uint32_t>(brx / kTile) + static_cast<uint32_t>(bry / kTile) * static_cast<uint32_t>(sc.ntx);
-      st_state(&st_out[gi], pack_state(best_t, best_ref));
+      if (!deferred) st_state(&st_out[gi], pack_state(best_t, best_ref));
       st_i32(&stamp[gi], k);
       ++cnt.improve;
     } else {
       st_state(&st_out[gi], pack_state(s.time[me], s.ref[me]));
+    }
+  }
+  if (kDefer) {
+    const unsigned long long dm = __ballot(deferred);
+    if (dm) {
+      const unsigned nd = static_cast<unsigned>(__builtin_popcountll(dm));
+      if (wq.n + nd > static_cast<unsigned>(kWalkQ)) {
+        const uint64_t tw = a.prof ? wall_clock64() : 0;
+        flush_walks(a, k, wq, cnt);
+        if (a.prof) walk_ticks += wall_clock64() - tw;
+      }
+      if (deferred) {
+        const unsigned slot = wq.n + static_cast<unsigned>(__builtin_popcountll(dm & ((1ull << lane) - 1ull)));
+        wq.q[slot] = WalkReq{si, static_cast<uint32_t>(gx) | (static_cast<uint32_t>(gy) << 16), best_ref_out,
+                             best_t_out};
+      }
+      wq.n += nd;
     }
   }
   uint64_t tp2 = a.prof ? wall_clock64() : 0;
@@ -680,6 +743,71 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   return changed;
 }
 
+// One list entry (scenario << 32 | tile, uniform over the wave): clears the
+// tile's mark, skips released scenarios and those past their sweep budget
+// (flagging the latter as capped), runs the tile.  Returns whether it
+// changed.
+template <bool kCache, bool kLdsBits>
+__device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned lds_off, int k, TileLds &s,
+                                 WalkQueue &wq, Counters &cnt) {
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev));
+  if (si >= static_cast<uint32_t>(a.nscen)) return false;
+  const Scen &sc = a.scen[si];
+  if (sc.max_iters == 0u) return false;  // released scenario: its buffers are gone
+  if (lane == 0) {
+    const uint32_t bit = 1u << (tile >> sc.nwords_log2);
+    and_relaxed(&mark_of(sc, k)[tile & ((1u << sc.nwords_log2) - 1u)], ~bit);
+  }
+  if (static_cast<unsigned>(k - sc.k0) >= sc.max_iters) {
+    if (lane == 0) st_u32(&a.out[si].capped, 1u);
+    return false;
+  }
+  if (tile >= static_cast<uint32_t>(sc.ntx) * static_cast<uint32_t>(sc.nty)) return false;
+  const bool tc = wave_tile<kCache, kLdsBits>(a, sc, si, lds_off, static_cast<int>(tile), k, s, wq, cnt);
+  if (tc && lane == 0) st_i32(&a.out[si].last_change, k);
+  return tc;
+}
+
+// Persistent form's share of one sweep: this workgroup's entries
+// blockIdx.x, blockIdx.x + gridDim.x, ... are handed to its waves one at a
+// time from an LDS counter (tiles differ widely in cost, so a fixed split
+// leaves waves idle at the barrier); the next entry's load is issued before
+// the current tile runs.  Entries fill slots from 0, so a wave stops at the
+// first empty slot it draws.
+template <bool kCache, bool kLdsBits>
+__device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds &s, WalkReq *walkq,
+                               Counters &cnt) {
+  WalkQueue wq{walkq, 0u};
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  unsigned long long *list = list_of(a.q, k);
+  auto draw = [&](size_t &e) {
+    unsigned i = 0;
+    if (lane == 0) i = atomicAdd(next, 1u);
+    i = __shfl(i, 0);
+    e = blockIdx.x + static_cast<size_t>(gridDim.x) * i;
+    return e < a.q.cap ? ld_u64(&list[e]) : kNoEntry;
+  };
+  bool changed = false;
+  size_t e;
+  unsigned long long v = draw(e);
+  while (v != kNoEntry) {
+    if (lane == 0) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
+    size_t e2;
+    const unsigned long long v2 = draw(e2);  // in flight while this tile runs
+    changed |= run_entry<kCache, kLdsBits>(a, v, lds_off, k, s, wq, cnt);
+    v = v2;
+    e = e2;
+  }
+  if (defer_walks<kCache, kLdsBits>()) {
+    const uint64_t tw = a.prof ? wall_clock64() : 0;
+    flush_walks(a, k, wq, cnt);
+    if (a.prof) cnt.ph[2] += wall_clock64() - tw;
+  }
+  return changed;
+}
+
 // One sweep's share of one wave: list entries gwave, gwave + nwaves, ...
 // (64 per round, one per lane), each slot emptied and its tile's mark
 // cleared after reading.  Entries fill slots from 0, so the first empty slot
@@ -688,7 +816,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
 // this wave changed.
 template <bool kCache, bool kLdsBits>
 __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds &s,
-                           Counters &cnt) {
+                           WalkReq *walkq, Counters &cnt) {
+  WalkQueue wq{walkq, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
   unsigned long long *list = list_of(a.q, k);
   bool changed = false;
@@ -701,26 +830,14 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
     for (unsigned long long r = m; r;) {
       const int l = __builtin_ctzll(r);
       r &= r - 1;
-      const unsigned long long ev = __shfl(v, l);
-      const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
-      const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev));
-      if (si >= static_cast<uint32_t>(a.nscen)) continue;
-      const Scen &sc = a.scen[si];
-      if (sc.max_iters == 0u) continue;  // released scenario: its buffers are gone
-      if (lane == 0) {
-        const uint32_t bit = 1u << (tile >> sc.nwords_log2);
-        and_relaxed(&mark_of(sc, k)[tile & ((1u << sc.nwords_log2) - 1u)], ~bit);
-      }
-      if (static_cast<unsigned>(k - sc.k0) >= sc.max_iters) {
-        if (lane == 0) st_u32(&a.out[si].capped, 1u);
-        continue;
-      }
-      if (tile >= static_cast<uint32_t>(sc.ntx) * static_cast<uint32_t>(sc.nty)) continue;
-      const bool tc = wave_tile<kCache, kLdsBits>(a, sc, si, lds_off, static_cast<int>(tile), k, s, cnt);
-      if (tc && lane == 0) st_i32(&a.out[si].last_change, k);
-      changed |= tc;
+      changed |= run_entry<kCache, kLdsBits>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
     }
     if (m != ~0ull) break;
+  }
+  if (defer_walks<kCache, kLdsBits>()) {
+    const uint64_t tw = a.prof ? wall_clock64() : 0;
+    flush_walks(a, k, wq, cnt);
+    if (a.prof) cnt.ph[2] += wall_clock64() - tw;
   }
   return changed;
 }
@@ -865,10 +982,13 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
 
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
-  unsigned long long v[8] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                             cnt.wait_ticks, 0ull, cnt.marks};
+  unsigned long long v[12] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
+                              cnt.wait_ticks, 0ull, cnt.marks, 0ull, 0ull, 0ull, 0ull};
+  // phase totals are per wave (lane 0's copy only)
+  const bool lead = (threadIdx.x & 63) == 0;
+  for (int q = 0; q < 4; ++q) v[8 + q] = lead ? cnt.ph_total[q] : 0ull;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
+  for (int q = 0; q < 12; ++q) {
     unsigned long long x = v[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(&a.ctl->stats[q], x);
@@ -877,38 +997,43 @@ __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
 
 // Persistent form: one launch runs sweeps k_begin.. until a sweep commits
 // nothing or k_end; workgroups meet at the grid barrier after each sweep.
-// Bitmap word j belongs to wave j mod (waves) with waves numbered
-// wave * gridDim.x + workgroup, so neighbouring tiles (which sit in
-// neighbouring words) land on different CUs.  Dynamic LDS: one TileLds per
-// wave, then (optionally) the boundary bitset.
+// List entry e belongs to workgroup e mod gridDim.x, so tiles queued
+// together (neighbours on the fire front) land on different CUs, and each
+// workgroup's waves draw its entries dynamically (sweep_wave_dyn).
 template <bool kCache, bool kLdsBits>
 __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
   // Per-sweep workgroup flags, double-buffered by sweep parity: sweep k
   // writes [k & 1], thread 0 clears it after barrier k, and it is written
   // again only in sweep k + 2, after barrier k + 1.
   __shared__ unsigned s_changed[2];
+  __shared__ unsigned s_next[2];  // entries drawn in sweep k (by parity, like s_changed)
   __shared__ unsigned long long s_ph[2][4];
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
+  // Dynamic LDS: nw tile slices, then (deferred walks) nw walk queues, then
+  // (LDS bitset) the boundary bitset.
   TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds);
   TileLds &s = slices[wave];
-  const unsigned lds_off = nw * static_cast<unsigned>(sizeof(TileLds) / 4);
+  constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
+  WalkReq *walkq = reinterpret_cast<WalkReq *>(slices + nw) + wave * kWalkQ;
+  const unsigned lds_off = nw * static_cast<unsigned>((sizeof(TileLds) + (kDefer ? sizeof(WalkReq) * kWalkQ : 0)) / 4);
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
+  if (threadIdx.x < 2) s_next[threadIdx.x] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   if (kLdsBits) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
   }
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
-  const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   for (int k = a.k_begin;; ++k) {
-    const bool changed = sweep_wave<kCache, kLdsBits>(a, gwave, nwaves, lds_off, k, s, cnt);
+    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, walkq, cnt);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         atomicMax(&s_ph[k & 1][q], static_cast<unsigned long long>(cnt.ph[q]));
+        cnt.ph_total[q] += cnt.ph[q];
         cnt.ph[q] = 0;
       }
     }
@@ -917,6 +1042,7 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
     const int done = grid_barrier(a, k, s_changed[k & 1] != 0u, t0, cnt, s_ph[k & 1]);
     if (threadIdx.x == 0) {
       s_changed[k & 1] = 0;
+      s_next[k & 1] = 0;
       t0 = wall_clock64();
     }
     if (threadIdx.x < 4) s_ph[k & 1][threadIdx.x] = 0;
@@ -929,14 +1055,15 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
 template <bool kCache>
 __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   __shared__ TileLds slices[256 / kWave];
+  __shared__ WalkReq walkq[256 / kWave][kWalkQ];
   __shared__ unsigned s_changed;
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
-  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], cnt);
+  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], walkq[wave], cnt);
   if ((threadIdx.x & (kWave - 1)) == 0 && changed) s_changed = 1u;
   __syncthreads();
   if (threadIdx.x == 0 && s_changed) add_relaxed(&a.ctl->changes.v[k & 1], 1u);
@@ -1362,10 +1489,13 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       a.lds_words = static_cast<unsigned>(L.lds_bits_words);
       waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / sizeof(TileLds));
     }
+    // per-wave LDS: the tile slice, plus a walk queue when walks are deferred
+    // (line-of-sight flags cached and the bitset read from L2)
+    const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(WalkReq) * kWalkQ : 0);
     const int max_waves = 16;  // the kernel's launch bound: 1024 threads
     if (tn.waves_per_group) waves = static_cast<int>(tn.waves_per_group);
     if (waves > max_waves) waves = max_waves;
-    const size_t shmem = waves * sizeof(TileLds) + (a.lds_bits ? bits_bytes : 0);
+    const size_t shmem = waves * wave_lds + (a.lds_bits ? bits_bytes : 0);
     const int block = waves * kWave;
     void (*kern)(Args) = L.cache ? (a.lds_bits ? propagate_persistent<true, true> : propagate_persistent<true, false>)
                                  : (a.lds_bits ? propagate_persistent<false, true> : propagate_persistent<false, false>);
@@ -1385,7 +1515,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     PG_HIP(hipStreamSynchronize(stream));
   }
   if (book[3] & 1u) return set_error(PG_ERR_TIMEOUT, "a grid barrier wait exceeded its spin limit");
-  unsigned long long counters[8];
+  unsigned long long counters[12];
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
   float rms = 0.0f;
@@ -1403,6 +1533,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     stats->dda_steps += counters[2];
     stats->tiles_visited += counters[3];
     stats->tile_marks += counters[7];
+    for (int q = 0; q < 4; ++q) stats->phase_total_ms[q] += static_cast<double>(counters[8 + q]) / clock_khz;
     stats->relax_ms += rms;
     stats->work_ms += static_cast<double>(counters[4]) / clock_khz / groups_d;
     stats->wait_ms += static_cast<double>(counters[5]) / clock_khz / groups_d;

@@ -63,12 +63,13 @@ class Stats(C.Structure):
                 ("crit_ms", C.c_double),
                 ("workgroups", C.c_uint32), ("lds_bitset", C.c_uint32),
                 ("phase_ms", C.c_double * 4), ("barrier_ms", C.c_double * 4),
-                ("tile_marks", C.c_uint64)]
+                ("tile_marks", C.c_uint64), ("phase_total_ms", C.c_double * 4)]
 
     def as_dict(self):
-        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("phase_ms", "barrier_ms")}
-        d["phase_ms"] = list(self.phase_ms)
-        d["barrier_ms"] = list(self.barrier_ms)
+        arrays = ("phase_ms", "barrier_ms", "phase_total_ms")
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f not in arrays}
+        for f in arrays:
+            d[f] = list(getattr(self, f))
         return d
 
 
