@@ -68,6 +68,72 @@ def ensemble_cases(seed, count, W, H, max_time):
     return cases
 
 
+def serve_stream(args, pp, synth, torch, tuning):
+    """Serving: Poisson arrivals over four resident landscapes of mixed sizes,
+    continuous batching through py_propag.serve.Server; latency = arrival to
+    arrival-time raster in (pinned) host memory."""
+    from py_propag.serve import Request, Server
+
+    wl = synth.serving_stream(seed=4, rate=args.serve_rate, seconds=args.serve_seconds)
+    sides = wl["sides"]
+    t0 = time.time()
+    lands = [wl["land"](i) for i in range(len(sides))]
+    log(f"serving: generated landscapes {sides} in {time.time() - t0:.1f}s; "
+        f"{len(wl['requests'])} requests over {args.serve_seconds:.0f} s")
+    geos = [pp.GeoReference(s, s, "+proj=utm +zone=30", [0.0, 5.0, 0.0, 0.0, 0.0, 5.0]) for s in sides]
+    ctx = pp.Context()
+    tids = [ctx.add_terrain(g, L["fuel"], L["slope"], L["aspect"]) for g, L in zip(geos, lands)]
+    pinned = torch.empty(max(sides) ** 2, dtype=torch.float32, pin_memory=True)
+    host = pinned.numpy()
+    check = {}
+    for i, c in enumerate(wl["requests"]):     # parity sample: the first request on each landscape
+        check.setdefault(c["terrain"], i)
+    check = set(check.values())
+    kept = {}
+
+    def on_done(req, sid):
+        n = sides[req.tag[1]] ** 2
+        ctx.time_into(sid, pinned.data_ptr())
+        if req.tag[0] in check:
+            kept[req.tag[0]] = host[:n].copy()
+
+    def requests(cases, warm=False):
+        return [Request(tids[c["terrain"]], c["wx"], c["ign"], c["max_time"], 0.0 if warm else c["arrival"],
+                        tag=(-1 if warm else i, c["terrain"])) for i, c in enumerate(cases)]
+
+    server = Server(ctx, on_done=on_done, tuning=tuning)
+    server.play(requests(wl["requests"][:8], warm=True))        # untimed: pools, code paths
+    server.launches, server.launch_ms = 0, 0.0
+    done = server.play(requests(wl["requests"]))
+    lat = np.array([(r.finished - r.arrival) * 1e3 for r in done])
+    span = max(r.finished for r in done) - min(r.arrival for r in done)
+    capped = sum(1 for r in done if r.state != pp.DONE)
+    ok = True
+    for i in sorted(check):
+        c = wl["requests"][i]
+        k = c["terrain"]
+        n = sides[k] ** 2
+        full = {"fuel": lands[k]["fuel"], "slope": lands[k]["slope"], "aspect": lands[k]["aspect"]}
+        for f in ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed", "wind_azimuth"):
+            full[f] = np.full(n, c["wx"][f], np.float32)
+        alone = pp.propagate(pp.Settings(geos[k], c["max_time"]), full, ignitions=c["ign"])
+        ok &= bool(np.array_equal(kept[i].view(np.uint32), alone.arrival_times.ravel().view(np.uint32)))
+    ctx.close()
+    out = {"value": len(done) / span, "unit": "scenarios/s", "requests": len(done),
+           "rate_per_s": args.serve_rate, "seconds": args.serve_seconds, "sides": sides,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+           "mean_in_flight": float(lat.sum() / 1e3 / span), "launches": server.launches,
+           "gpu_launch_busy": server.launch_ms / 1e3 / span, "not_converged": capped,
+           "busy_refusals": server.busy_refusals, "parity_sample": len(check), "parity": ok,
+           "workload": f"Poisson {args.serve_rate:g}/s for {args.serve_seconds:g} s over 4 landscapes "
+                       f"{sides} at 5 m, max_time U(3600,36000) s, 1-4 ignitions, uniform wind"}
+    log(f"served: {out['value']:.1f} scenarios/s, latency p50 {out['p50_ms']:.2f} p99 {out['p99_ms']:.2f} "
+        f"max {out['max_ms']:.2f} ms, {out['mean_in_flight']:.2f} in flight, {server.launches} launches, "
+        f"GPU launches {100 * out['gpu_launch_busy']:.0f}% of the time; parity on {len(check)} requests: {ok}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +145,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-phases", action="store_true",
                     help="time the sweep phases inside the kernel (slower; diagnostics only)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the served Poisson request stream (0: skip serving)")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (rocprof counter scaling)")
     args = ap.parse_args()
@@ -214,6 +283,7 @@ def main():
         f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
         f"phase totals over waves {es['phase_total_ms']}, "
         f"barrier (start/span/gather/book) {es['barrier_ms']}")
+    serving = serve_stream(args, pp, synth, torch, tuning) if args.serve_seconds > 0 else None
     ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
     alg_bytes = 24.0 * es["relaxations"] + 8.0 * es["improvements"]
     achieved = alg_bytes / (ens_relax_ms * 1e-3) / 1e9
@@ -299,6 +369,7 @@ def main():
         "rothermel": {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,
                       "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS,
                       "workload": "rothermel_field_1024"},
+        "serving": serving,
         "cpu_baseline": cpu,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
                                                        "tiles_visited", "tile_marks", "relax_ms", "elapsed_ms",

@@ -224,6 +224,13 @@ pg_status pg_scenario_admit(pg_ctx *ctx, int terrain_id, const pg_weather *weath
 pg_status pg_batch_run(pg_ctx *ctx, uint32_t max_sweeps, const pg_tuning *tuning,
                        pg_stats *stats, uint32_t *n_finished);
 
+/* Drains the ids of scenarios that finished (done or capped) in launches
+ * since the previous call, oldest first: at most `capacity` ids go to
+ * `ids`, *count receives how many; the rest stay for the next call.
+ * Released scenarios are dropped from the list.  A serving loop calls this
+ * after each pg_batch_run instead of polling every scenario. */
+pg_status pg_ctx_take_finished(pg_ctx *ctx, int *ids, uint32_t capacity, uint32_t *count);
+
 /* State of a scenario and the sweeps it has run. */
 pg_status pg_scenario_status(pg_ctx *ctx, int scenario_id, int *state, uint64_t *sweeps);
 

@@ -90,6 +90,8 @@ struct pg_ctx {
   std::vector<void *> dev_ign;
   // Freed scenario blocks by size, reused before allocating.
   std::multimap<size_t, void *> pool;
+  // Scenarios finished since the last pg_ctx_take_finished, oldest first.
+  std::vector<int> finished;
   int k_next = 0;
 };
 
@@ -490,6 +492,7 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
     st = scen_finish(s.sc, scen_final_buf(s.sc, o), s.speed, s.time, s.rx, s.ry, c->stream);
     if (st != PG_OK) return st;
     s.posted = true;
+    c->finished.push_back(i);
     ++fin;
   }
   c->k_next = res.last_sweep + 1;
@@ -504,6 +507,16 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
     stats->elapsed_ms = ms;
   }
   if (n_finished) *n_finished = fin;
+  return PG_OK;
+}
+
+pg_status pg_ctx_take_finished(pg_ctx *c, int *ids, uint32_t capacity, uint32_t *count) {
+  if (!c || !count || (!ids && capacity)) return set_error(PG_ERR_INVALID, "pg_ctx_take_finished: null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const size_t n = std::min<size_t>(capacity, c->finished.size());
+  std::copy(c->finished.begin(), c->finished.begin() + n, ids);
+  c->finished.erase(c->finished.begin(), c->finished.begin() + n);
+  *count = static_cast<uint32_t>(n);
   return PG_OK;
 }
 
@@ -541,6 +554,7 @@ pg_status pg_scenario_release(pg_ctx *c, int id) {
     return set_error(PG_ERR_INVALID, "unknown scenario id");
   Scenario &s = c->scen[id];
   const bool running = s.state == PG_SCENARIO_RUNNING;
+  c->finished.erase(std::remove(c->finished.begin(), c->finished.end(), id), c->finished.end());
   pg_status st = put_descriptor(c, id, dead_descriptor_ptr());
   if (st != PG_OK) return st;
   // Work on the context's stream is ordered: the block can be handed to a

@@ -539,6 +539,21 @@ class Context:
                    "pg_scenario_result")
         return time.reshape(h, w), rx, ry, bits
 
+    def time_into(self, sid: int, address: int):
+        """Copies a finished scenario's arrival times (W*H f32) to host memory
+        at ``address`` (for instance a pinned buffer) and waits for it."""
+        _raise_for(self._lib.pg_scenario_result(self._h, int(sid), C.c_void_p(address), None, None, None),
+                   "pg_scenario_result")
+
+    def take_finished(self, capacity: int = 4096):
+        """Ids of scenarios that finished since the previous call, oldest
+        first."""
+        ids = (C.c_int * capacity)()
+        n = C.c_uint32()
+        _raise_for(self._lib.pg_ctx_take_finished(self._h, ids, C.c_uint32(capacity), C.byref(n)),
+                   "pg_ctx_take_finished")
+        return list(ids[:n.value])
+
     def release(self, sid: int):
         _raise_for(self._lib.pg_scenario_release(self._h, int(sid)), "pg_scenario_release")
         self._scen.pop(sid, None)

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpropag_hip.so")
 EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propagate",
            "pg_propagate_host", "pg_ctx_create", "pg_ctx_destroy", "pg_terrain_add",
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
-           "pg_scenario_result", "pg_scenario_release")
+           "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished")
 
 
 class GT(C.Structure):
@@ -116,7 +116,8 @@ def load(require_torch: bool = True):
                        ("pg_batch_run", [vp, u32, C.POINTER(Tuning), C.POINTER(Stats), C.POINTER(u32)]),
                        ("pg_scenario_status", [vp, i32, C.POINTER(i32), C.POINTER(C.c_uint64)]),
                        ("pg_scenario_result", [vp, i32, vp, vp, vp, vp]),
-                       ("pg_scenario_release", [vp, i32])):
+                       ("pg_scenario_release", [vp, i32]),
+                       ("pg_ctx_take_finished", [vp, C.POINTER(i32), u32, C.POINTER(u32)])):
         f = getattr(lib, name)
         f.restype = C.c_int
         f.argtypes = args

@@ -14,6 +14,8 @@ Workloads:
   large_scenario    4096x4096, 80 hills, 3000 patches, 16 points + 200-cell line, 12 h
   ensemble          scenarios of uneven wind / moisture / ignitions / max_time
                     over one shared landscape
+  serving_stream    Poisson arrivals of such scenarios over several resident
+                    landscapes of mixed raster sizes
 """
 from __future__ import annotations
 
@@ -236,3 +238,47 @@ def uniform_scenario(side=1024, fuel=1, moisture=0.1, wind=5.0, wind_az=0.0, slo
     return {"terrain": terrain, "width": side, "height": side, "dx": cell, "dy": cell,
             "max_time": np.float32(max_time), "time": time, "refs_x": rx, "refs_y": ry,
             "ignitions": [(ignition[0], ignition[1], 0.0)]}
+
+
+def scenario_weather(r, ign_count, w, h, max_time, margin=8):
+    """Weather and ignitions of one ensemble / serving scenario: 1-h moisture
+    U(0.03,0.12), 10-h/100-h +0.01/+0.02, herb U(0.3,1.5), wood U(0.6,1.5),
+    uniform wind U(2,12) m/s toward U(0,360) deg, ``ign_count`` distinct
+    point ignitions at t = 0 inside the central (1 - 2/margin) of the raster."""
+    d1 = np.float32(r.uniform(0.03, 0.12))
+    wx = {"d1hr": d1, "d10hr": np.float32(d1 + np.float32(0.01)),
+          "d100hr": np.float32(d1 + np.float32(0.02)), "herb": np.float32(r.uniform(0.3, 1.5)),
+          "wood": np.float32(r.uniform(0.6, 1.5)), "wind_speed": np.float32(r.uniform(2.0, 12.0)),
+          "wind_azimuth": np.float32(np.radians(r.uniform(0.0, 360.0)))}
+    cells = set()
+    while len(cells) < ign_count:
+        cells.add((int(r.integers(w // margin, w - w // margin)), int(r.integers(h // margin, h - h // margin))))
+    return {"wx": wx, "ign": [(x, y, 0.0) for x, y in sorted(cells)], "max_time": float(max_time)}
+
+
+def serving_stream(seed: int = 4, rate: float = 50.0, seconds: float = 60.0, terrains: int = 4,
+                   side_lo: int = 512, side_hi: int = 4096) -> dict:
+    """Serving workload: ``terrains`` landscapes with sides log-uniform in
+    [side_lo, side_hi] (config-1 generator, hills scaled with the side and
+    fuel patches with the area as between the 1024 and 4096 landscapes:
+    20 hills and 200 patches at 1024), and Poisson arrivals at ``rate`` per second
+    for ``seconds``; each request picks a landscape uniformly, draws its
+    weather and 1-4 ignitions as ``scenario_weather`` and max_time
+    U(3600, 36000) s.  Landscapes are generated lazily by ``land(i)``."""
+    r = _rng(seed)
+    sides = [int(round(math.exp(r.uniform(math.log(side_lo), math.log(side_hi))))) for _ in range(terrains)]
+    t, reqs = 0.0, []
+    while True:
+        t += r.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        k = int(r.integers(0, terrains))
+        case = scenario_weather(r, int(r.integers(1, 5)), sides[k], sides[k], r.uniform(3600.0, 36000.0))
+        case.update({"arrival": t, "terrain": k})
+        reqs.append(case)
+
+    def land(i):
+        a = sides[i] / 1024.0
+        return landscape(seed * 1000 + i, sides[i], max(1, round(20 * a)), max(4, round(200 * a * a)))
+
+    return {"sides": sides, "requests": reqs, "land": land, "rate": rate, "seconds": seconds}

@@ -236,3 +236,40 @@ def test_single_scenario_1024_through_the_context():
         sid = ctx.admit(tid, wx, sc["ignitions"], float(sc["max_time"]))
         ctx.run()
         _same(ctx.result(sid), _alone(pp, land, w, h, wx, sc["ignitions"], float(sc["max_time"])))
+
+
+def test_server_stream_each_result_same_as_alone():
+    """The request loop: arrivals spread in time over two landscapes, launches
+    of a few sweeps, a tight memory budget (admission waits for releases);
+    every delivered result equals the one-scenario path's."""
+    import py_propag as pp
+    from py_propag.serve import Request, Server
+
+    shapes = [(120, 120), (72, 200)]
+    lands = [_terrain(70 + j, w, h) for j, (w, h) in enumerate(shapes)]
+    cases = []
+    for s in range(12):
+        j = s % 2
+        w, h = shapes[j]
+        c = _case(80 + s, w, h)
+        c.update({"terrain": j, "arrival": 0.004 * s})
+        cases.append(c)
+    budget = 4 * max(pp.Context.scenario_bytes(w, h) for w, h in shapes)
+    got = {}
+    with pp.Context(hbm_budget=budget) as ctx:
+        tids = [ctx.add_terrain(_geo(pp, w, h), L["fuel"], L["slope"], L["aspect"]) for (w, h), L in zip(shapes, lands)]
+
+        def on_done(req, sid):
+            got[req.tag] = ctx.result(sid)
+
+        server = Server(ctx, on_done=on_done, target_ms=0.2, min_sweeps=4)
+        reqs = [Request(tids[c["terrain"]], c["wx"], c["ign"], c["max_time"], c["arrival"], tag=i)
+                for i, c in enumerate(cases)]
+        done = server.play(reqs)
+        assert len(done) == len(cases) and server.idle()
+        assert server.launches > 2
+        assert ctx.take_finished() == []
+    for i, c in enumerate(cases):
+        w, h = shapes[c["terrain"]]
+        _same(got[i], _alone(pp, lands[c["terrain"]], w, h, c["wx"], c["ign"], c["max_time"]))
+    print(f"served {len(cases)} requests in {server.launches} launches, {server.busy_refusals} refusals")
