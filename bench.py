@@ -68,6 +68,123 @@ def ensemble_cases(seed, count, W, H, max_time):
     return cases
 
 
+def device_columns(torch, dev, terrain, n, fields):
+    return [torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(terrain[k], np.float32), (n,))))
+            .to(dev) for k in fields]
+
+
+def plain_run(pp, torch, dev, stream, settings, fuel, cols, init, tuning):
+    """One scenario through the drop-in form: pg_burn + pg_propagate on
+    resident columns; returns (wall ms, stats, time tensor)."""
+    n = fuel.numel()
+    fire = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    t, rx, ry = (x.clone() for x in init)
+    bits = torch.zeros((n + 31) // 32, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    pp.burn_device(fuel, *cols, *fire, stream=stream)
+    st = pp.propagate_device(settings, *fire, t, rx, ry, bits, tuning=tuning, stream=stream)
+    return (time.perf_counter() - s0) * 1e3, st, t
+
+
+def large_single(args, pp, synth, torch, tuning, stream):
+    """The 4096x4096 12-h scenario (16 point ignitions + a 200-cell line),
+    plain form, resident inputs."""
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    sc = synth.large_scenario(seed=2, side=4096)
+    W, H = sc["width"], sc["height"]
+    n = W * H
+    log(f"large: generated 4096^2 scenario in {time.time() - t0:.1f}s")
+    g = pp.GeoReference(W, H, "+proj=utm +zone=30", [0.0, sc["dx"], 0.0, 0.0, 0.0, sc["dy"]])
+    settings = pp.Settings(g, float(sc["max_time"]))
+    fuel = torch.from_numpy(sc["terrain"]["fuel"]).to(dev)
+    cols = device_columns(torch, dev, sc["terrain"], n, FIELDS)
+    init = (torch.from_numpy(sc["time"]).to(dev), torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev),
+            torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev))
+    plain_run(pp, torch, dev, stream, settings, fuel, cols, init, tuning)
+    runs = [plain_run(pp, torch, dev, stream, settings, fuel, cols, init, tuning) for _ in range(3)]
+    ms = [r[0] for r in runs]
+    st, t = runs[-1][1], runs[-1][2]
+    burned = int((t < pp.FLT_MAX).sum().item())
+    same = all(torch.equal(r[2], t) for r in runs)
+    out = {"workload": "large_scenario_4096 (16 points + 200-cell line, 12 h)", "ms": float(np.mean(ms)),
+           "ms_min": float(np.min(ms)), "relax_ms": float(np.mean([r[1]["relax_ms"] for r in runs])),
+           "sweeps": st["iterations"], "relaxations": st["relaxations"], "improvements": st["improvements"],
+           "tiles_visited": st["tiles_visited"], "dda_steps": st["dda_steps"], "burned_cells": burned,
+           "repeatable": bool(same), "ignitions": len(sc["ignitions"])}
+    log(f"large 4096^2: {out['ms']:.2f} ms per scenario (relax {out['relax_ms']:.2f} ms), {out['sweeps']} sweeps, "
+        f"{burned} burned, {out['tiles_visited']} tile visits; repeatable {same}")
+    return out
+
+
+def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
+    """256 uneven scenarios on one 2048x2048 landscape in one shared launch,
+    beside a sample of them one per call."""
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    wl = synth.ensemble_2048(seed=3, count=args.batch2048)
+    side = wl["side"]
+    n = side * side
+    land = wl["land"]
+    log(f"ensemble 2048: generated in {time.time() - t0:.1f}s")
+    g = pp.GeoReference(side, side, "+proj=utm +zone=30", [0.0, 5.0, 0.0, 0.0, 0.0, 5.0])
+    ctx = pp.Context()
+    tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+
+    def ensemble():
+        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in wl["cases"]]
+        if any(i is None for i in ids):
+            raise RuntimeError("ensemble does not fit the memory budget")
+        fin, stats = ctx.run(tuning=tuning)
+        if fin != len(ids):
+            raise RuntimeError(f"{len(ids) - fin} scenarios did not finish")
+        return ids, stats
+
+    ids, _ = ensemble()
+    for i in ids:
+        ctx.release(i)
+    torch.cuda.synchronize()
+    ms, stats = [], None
+    for k in range(2):
+        s0 = time.perf_counter()
+        ids, stats = ensemble()
+        ms.append((time.perf_counter() - s0) * 1e3)
+        if k == 1:
+            kept = [ctx.result(ids[i])[0].ravel().copy() for i in range(2)]
+        for i in ids:
+            ctx.release(i)
+    ctx.close()
+    value = len(wl["cases"]) / (np.mean(ms) / 1e3)
+    # a sample one per call, drop-in form
+    fuel = torch.from_numpy(land["fuel"]).to(dev)
+    slope = torch.from_numpy(land["slope"]).to(dev)
+    aspect = torch.from_numpy(land["aspect"]).to(dev)
+    plain_ms, parity = [], True
+    for i, c in enumerate(wl["cases"][:args.plain2048]):
+        cols = [slope if k == "slope" else aspect if k == "aspect" else
+                torch.full((n,), float(c["wx"][k]), dtype=torch.float32, device=dev) for k in FIELDS]
+        tm, rx, ry = synth.initial_state(side, side, c["ign"])
+        init = (torch.from_numpy(tm).to(dev), torch.from_numpy(rx.view(np.int16)).to(dev),
+                torch.from_numpy(ry.view(np.int16)).to(dev))
+        wall, _, t = plain_run(pp, torch, dev, stream, pp.Settings(g, c["max_time"]), fuel, cols, init, tuning)
+        plain_ms.append(wall)
+        if i < len(kept):
+            parity &= bool(np.array_equal(t.cpu().numpy().view(np.uint32), kept[i].view(np.uint32)))
+    plain_sps = 1e3 / float(np.mean(plain_ms[1:] if len(plain_ms) > 1 else plain_ms))
+    out = {"workload": f"ensemble of {len(wl['cases'])} scenarios on one 2048x2048 landscape, max_time "
+                       "log-uniform 3600-43200 s, 1-4 ignitions, uniform wind U(2,12) m/s",
+           "value": value, "unit": "scenarios/s", "ms_per_ensemble": float(np.mean(ms)),
+           "relax_ms": stats["relax_ms"], "sweeps": stats["iterations"], "tiles_visited": stats["tiles_visited"],
+           "relaxations": stats["relaxations"], "dda_steps": stats["dda_steps"],
+           "plain_value": plain_sps, "plain_sample": len(plain_ms), "speedup_batched": value / plain_sps,
+           "parity_first_scenarios": parity}
+    log(f"ensemble 2048: {out['ms_per_ensemble']:.1f} ms per ensemble of {len(wl['cases'])} ({value:.1f} "
+        f"scenarios/s; one per call {plain_sps:.1f}/s, {value / plain_sps:.1f}x), {stats['iterations']} sweeps, "
+        f"parity {parity}")
+    return out
+
+
 def serve_stream(args, pp, synth, torch, tuning):
     """Serving: Poisson arrivals over four resident landscapes of mixed sizes,
     continuous batching through py_propag.serve.Server; latency = arrival to
@@ -148,6 +265,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the served Poisson request stream (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
+    ap.add_argument("--batch2048", type=int, default=256, help="scenarios of the 2048x2048 ensemble (0: skip)")
+    ap.add_argument("--plain2048", type=int, default=6, help="of which timed one per call")
+    ap.add_argument("--no-large", action="store_true", help="skip the 4096x4096 scenario")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (rocprof counter scaling)")
     args = ap.parse_args()
@@ -284,6 +404,8 @@ def main():
         f"phase totals over waves {es['phase_total_ms']}, "
         f"barrier (start/span/gather/book) {es['barrier_ms']}")
     serving = serve_stream(args, pp, synth, torch, tuning) if args.serve_seconds > 0 else None
+    ens2048 = ensemble_2048_section(args, pp, synth, torch, tuning, stream) if args.batch2048 > 0 else None
+    large = None if args.no_large else large_single(args, pp, synth, torch, tuning, stream)
     ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
     alg_bytes = 24.0 * es["relaxations"] + 8.0 * es["improvements"]
     achieved = alg_bytes / (ens_relax_ms * 1e-3) / 1e9
@@ -370,6 +492,8 @@ def main():
                       "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS,
                       "workload": "rothermel_field_1024"},
         "serving": serving,
+        "ensemble_2048": ens2048,
+        "large_4096": large,
         "cpu_baseline": cpu,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
                                                        "tiles_visited", "tile_marks", "relax_ms", "elapsed_ms",

@@ -335,14 +335,18 @@ struct Args {
   unsigned prof_sweeps;      // rows of prof
 };
 
+constexpr int kPairsMax = kTile * kTile * 8;  // (cell, neighbour) pairs of one tile
+
 struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
+  unsigned long long key[kTile * kTile];  // per cell: least candidate key (packed rule)
   float time[kHaloCells];
   uint32_t ref[kHaloCells];
   float rtime[kHaloCells];
   float ca[kHaloCells], cb[kHaloCells], cr[kHaloCells];  // fire coefficients A, B, R (R = inf: null fire)
-  uint8_t chg[kHaloCells];
+  uint16_t pair[kPairsMax];  // packed rule: cell << 4 | neighbour
+  uint8_t chg[kHaloCells];   // bit 0: changed in sweep k - 1; bit 1: also feeds the rule
 };
-static_assert(sizeof(TileLds) % 4 == 0, "TileLds must keep 4-byte alignment");
+static_assert(sizeof(TileLds) % 8 == 0, "TileLds must keep 8-byte alignment");
 
 
 // Per-workgroup arrival slot of the grid barrier, one 128-byte line each.
@@ -511,6 +515,101 @@ __device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_
   return 9u + static_cast<uint32_t>(nt.n);
 }
 
+// Order-preserving map of a float to 32 bits (negative values below
+// positive ones) and back.
+__device__ inline uint32_t order_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float from_order_bits(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// The update rule (find_neighbor_which_reaches_first, propag.cu:124-190)
+// with line-of-sight flags cached in the references.
+//
+// A neighbour n offers the candidate cand(n) = ct if ct < nt and the line
+// from n to its reference is clear, else nt; the rule keeps the first
+// neighbour in its order (j outer, i inner) whose candidate is below the
+// running best, i.e. the least (cand, order) pair if that candidate beats
+// the cell's own time.  cand(n) depends only on n's state, the live time of
+// n's reference and static data, so a neighbour whose state and reference
+// time are unchanged since the cell's previous evaluation offers the same
+// candidate as then, which was no better than the cell's time then and
+// cannot be better than its (never larger) time now.  Only neighbours
+// changed in sweep k - 1 (chg bit 0, stamps) that can feed the rule (bit 1)
+// are evaluated: their (cell, neighbour) pairs are packed over the wave's
+// lanes and each cell's least key (order-mapped time, neighbour index,
+// reference-or-neighbour bit) is an LDS atomic minimum.
+__device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, int lx, int ly, int gx, int gy,
+                                   int lane, bool need, Counters &cnt, bool &improved, float &best_t_out,
+                                   uint32_t &best_ref_out) {
+  uint32_t m = 0;
+  if (need) {
+    ++cnt.relax;
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      if (d == 4) continue;
+      const int i = d % 3 - 1, j = d / 3 - 1;
+      const int nx = gx + i, ny = gy + j;
+      const bool inb = nx >= 0 && ny >= 0 && nx < g.W && ny < g.H;
+      m |= (inb && s.chg[(lx + i) + (ly + j) * kHalo] == 3) ? (1u << d) : 0u;
+    }
+  }
+  // Exclusive wave scan of the pair counts (0..8), one ballot per bit.
+  const uint32_t c = static_cast<uint32_t>(__builtin_popcount(m));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t excl = 0, total = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const unsigned long long bb = __ballot((c >> b) & 1u);
+    excl += static_cast<uint32_t>(__builtin_popcountll(bb & below)) << b;
+    total += static_cast<uint32_t>(__builtin_popcountll(bb)) << b;
+  }
+  s.key[lane] = ~0ull;
+  for (uint32_t mm = m, pos = excl; mm; mm &= mm - 1u, ++pos)
+    s.pair[pos] = static_cast<uint16_t>((lane << 4) | __builtin_ctz(mm));
+  wave_sync();
+  for (uint32_t p0 = 0; p0 < total; p0 += kWave) {
+    const uint32_t e = p0 + static_cast<uint32_t>(lane);
+    if (e < total) {
+      const uint32_t pr = s.pair[e];
+      const int cl = static_cast<int>(pr >> 4), d = static_cast<int>(pr & 15u);
+      const int cx = (cl & (kTile - 1)) + 1, cy = cl / kTile + 1;
+      const int tx = ox + cx - 1, ty = oy + cy - 1;
+      const int i = d % 3 - 1, j = d / 3 - 1;
+      const int nl = (cx + i) + (cy + j) * kHalo;
+      const uint32_t ref = s.ref[nl];
+      const int rfx = ref_x<true>(ref), rfy = ref_y<true>(ref);
+      if (rfx != tx || rfy != ty) {  // a neighbour referencing this cell offers nothing
+        const float4 kf = make_float4(s.ca[nl], s.cb[nl], s.cr[nl], 0.0f);
+        const float nt = time_from(s.time[nl], tx + i, ty + j, tx, ty, kf, g);
+        const float ct = time_from(s.rtime[nl], rfx, rfy, tx, ty, kf, g);
+        const bool use_ref = ct < nt && (ref & kClearBit) != 0u;
+        const float cand = use_ref ? ct : nt;
+        const unsigned long long key = (static_cast<unsigned long long>(order_bits(cand)) << 5) |
+                                       static_cast<unsigned long long>(d << 1) | (use_ref ? 1ull : 0ull);
+        atomicMin(&s.key[cl], key);
+      }
+    }
+  }
+  wave_sync();
+  if (need) {
+    const int me = lx + ly * kHalo;
+    const float own_t = s.time[me];
+    const unsigned long long key = s.key[lane];
+    const float cand = from_order_bits(static_cast<uint32_t>(key >> 5));
+    improved = key != ~0ull && cand < own_t;  // update_point (propag.cu:192-200)
+    if (improved) {
+      const int d = static_cast<int>((key >> 1) & 15u);
+      const int i = d % 3 - 1, j = d / 3 - 1;
+      const uint32_t ref = s.ref[(lx + i) + (ly + j) * kHalo];
+      best_t_out = cand;
+      best_ref_out = (key & 1u) ? make_ref(ref_x<true>(ref), ref_y<true>(ref)) : make_ref(gx + i, gy + j);
+    }
+  }
+}
+
 // One sweep over one tile by one wave: lane l owns cell (l % 8, l / 8) of
 // the 8x8 tile.  Returns, uniformly over the wave, whether any cell
 // committed an improvement.
@@ -577,7 +676,10 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       s.ca[h] = cf[it].x;  // Coef: a, b, r, null_fire
       s.cb[h] = cf[it].y;
       s.cr[h] = cf[it].w != 0.0f ? __builtin_inff() : cf[it].z;
-      s.chg[h] = (stp[it] >= k - 1 || rstp[it] >= k - 1) ? 1 : 0;
+      const float t = __uint_as_float(w[it].x);
+      const bool chg = stp[it] >= k - 1 || rstp[it] >= k - 1;
+      const bool src = t < g.max_time && cf[it].w == 0.0f;  // burns before max_time, fire not null
+      s.chg[h] = static_cast<uint8_t>((chg ? 1 : 0) | (chg && src ? 2 : 0));
     }
   }
   wave_sync();
@@ -591,7 +693,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
 #pragma unroll
     for (int j = -1; j < 2; ++j)
 #pragma unroll
-      for (int i = -1; i < 2; ++i) need = need || s.chg[(lx + i) + (ly + j) * kHalo];
+      for (int i = -1; i < 2; ++i) need = need || (s.chg[(lx + i) + (ly + j) * kHalo] & 1);
   }
   bool improved = false;
   bool deferred = false;  // improved with a new reference whose walk is queued
@@ -599,7 +701,9 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   uint32_t rtile = 0;    // that reference's tile
   float best_t_out = 0.0f;
   uint32_t best_ref_out = 0u;
-  if (need) {
+  if constexpr (kCache) {
+    packed_rule(s, g, ox, oy, lx, ly, gx, gy, lane, need, cnt, improved, best_t_out, best_ref_out);
+  } else if (need) {
     ++cnt.relax;
     const int me = lx + ly * kHalo;
     const float own_t = s.time[me];

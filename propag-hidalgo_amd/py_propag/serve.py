@@ -58,7 +58,9 @@ class Server:
         self.queue = collections.deque()
         self.running = {}            # scenario id -> Request
         self.launches = 0
-        self.launch_ms = 0.0
+        self.launch_ms = 0.0         # host wall time inside launches
+        self.kernel_ms = 0.0         # relaxation kernel time (HIP events)
+        self.sweeps_run = 0
         self.busy_refusals = 0
 
     def submit(self, req: Request):
@@ -87,10 +89,12 @@ class Server:
         if not self.running:
             return []
         t0 = self.clock()
-        self.ctx.run(max_sweeps=self.sweeps, tuning=self.tuning)
+        _, stats = self.ctx.run(max_sweeps=self.sweeps, tuning=self.tuning)
         ms = (self.clock() - t0) * 1e3
         self.launches += 1
         self.launch_ms += ms
+        self.kernel_ms += stats["relax_ms"]
+        self.sweeps_run += stats["iterations"]
         if ms > 0:
             want = int(self.sweeps * self.target_ms / ms)
             self.sweeps = max(self.min_sweeps, min(self.max_sweeps, (self.sweeps + want) // 2))

@@ -282,3 +282,18 @@ def serving_stream(seed: int = 4, rate: float = 50.0, seconds: float = 60.0, ter
         return landscape(seed * 1000 + i, sides[i], max(1, round(20 * a)), max(4, round(200 * a * a)))
 
     return {"sides": sides, "requests": reqs, "land": land, "rate": rate, "seconds": seconds}
+
+
+def ensemble_2048(seed: int = 3, count: int = 256, side: int = 2048) -> dict:
+    """Batched-ensemble workload: ``count`` scenarios sharing one
+    ``side``-square landscape (config-1 generator scaled as in
+    serving_stream), each with its own weather and 1-4 ignitions as
+    ``scenario_weather`` and max_time log-uniform in [3600, 43200] s."""
+    r = _rng(seed)
+    cases = []
+    for _ in range(count):
+        mt = math.exp(r.uniform(math.log(3600.0), math.log(43200.0)))
+        cases.append(scenario_weather(r, int(r.integers(1, 5)), side, side, mt))
+    a = side / 1024.0
+    land = landscape(seed * 1000, side, max(1, round(20 * a)), max(4, round(200 * a * a)))
+    return {"side": side, "land": land, "cases": cases}
