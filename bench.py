@@ -68,6 +68,25 @@ def ensemble_cases(seed, count, W, H, max_time):
     return cases
 
 
+def autotuned(args, ctx, tid, cases, W, H, tuning):
+    """The ensemble's launch tuning: the autotuner's pick on a sample of
+    --tune-sample scenarios (or the fixed bench tuning with --no-autotune /
+    --profile-phases)."""
+    if args.no_autotune or args.profile_phases or args.mode == 1:
+        return dict(tuning), None
+    from py_propag.autotune import Autotuner
+
+    t0 = time.perf_counter()
+    tuner = Autotuner(ctx)
+    best = tuner.tune(tid, cases[:args.tune_sample], W, H, count=len(cases))
+    table = tuner.tables[tuner.key(W, H, len(cases))]
+    log(f"autotune ({len(table)} candidates on {min(len(cases), args.tune_sample)} scenarios, "
+        f"{time.perf_counter() - t0:.1f}s): best {best}; "
+        + ", ".join(f"{t['waves_per_group']}w/{t['workgroups']}g/m{t['mode']} {dt * 1e3:.1f}ms" for t, dt in table))
+    return dict(best), {"best": best, "sample": min(len(cases), args.tune_sample),
+                        "table": [{"tuning": t, "ms": dt * 1e3} for t, dt in table]}
+
+
 def device_columns(torch, dev, terrain, n, fields):
     return [torch.from_numpy(np.ascontiguousarray(np.broadcast_to(np.asarray(terrain[k], np.float32), (n,))))
             .to(dev) for k in fields]
@@ -131,12 +150,13 @@ def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
     g = pp.GeoReference(side, side, "+proj=utm +zone=30", [0.0, 5.0, 0.0, 0.0, 0.0, 5.0])
     ctx = pp.Context()
     tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+    btuning, tune = autotuned(args, ctx, tid, wl["cases"], side, side, tuning)
 
     def ensemble():
         ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in wl["cases"]]
         if any(i is None for i in ids):
             raise RuntimeError("ensemble does not fit the memory budget")
-        fin, stats = ctx.run(tuning=tuning)
+        fin, stats = ctx.run(tuning=btuning)
         if fin != len(ids):
             raise RuntimeError(f"{len(ids) - fin} scenarios did not finish")
         return ids, stats
@@ -178,7 +198,7 @@ def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
            "relax_ms": stats["relax_ms"], "sweeps": stats["iterations"], "tiles_visited": stats["tiles_visited"],
            "relaxations": stats["relaxations"], "dda_steps": stats["dda_steps"],
            "plain_value": plain_sps, "plain_sample": len(plain_ms), "speedup_batched": value / plain_sps,
-           "parity_first_scenarios": parity}
+           "parity_first_scenarios": parity, "autotune": tune}
     log(f"ensemble 2048: {out['ms_per_ensemble']:.1f} ms per ensemble of {len(wl['cases'])} ({value:.1f} "
         f"scenarios/s; one per call {plain_sps:.1f}/s, {value / plain_sps:.1f}x), {stats['iterations']} sweeps, "
         f"parity {parity}")
@@ -260,6 +280,8 @@ def main():
     ap.add_argument("--plain", type=int, default=16, help="ensemble scenarios timed one per call")
     ap.add_argument("--mode", type=int, default=0, help="0 persistent kernel, 1 one launch per sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="run the ensembles with the default tuning")
+    ap.add_argument("--tune-sample", type=int, default=32, help="scenarios per autotuner trial")
     ap.add_argument("--profile-phases", action="store_true",
                     help="time the sweep phases inside the kernel (slower; diagnostics only)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -366,12 +388,13 @@ def main():
     # ------------------------------------------------ batched: ensembles
     ctx = pp.Context()
     tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+    btuning, tune_1024 = autotuned(args, ctx, tid, cases, W, H, tuning)
 
     def ensemble():
         ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
         if any(i is None for i in ids):
             raise RuntimeError("ensemble does not fit the memory budget")
-        fin, stats = ctx.run(tuning=tuning)
+        fin, stats = ctx.run(tuning=btuning)
         return ids, fin, stats
 
     for _ in range(args.warmup):
@@ -475,7 +498,8 @@ def main():
         "config": {"workload": f"ensemble of {args.batch} 1024x1024 10-h scenarios on the config-1 landscape",
                    "cells_per_scenario": n, "cell_m": 5.0, "max_time_s": 36000.0,
                    "form": "batched (one shared persistent launch per ensemble)",
-                   "schedule": "persistent" if args.mode == 0 else "one launch per sweep"},
+                   "schedule": "one launch per sweep" if args.mode == 1 else "persistent",
+                   "tuning": btuning},
         "latency_ms": {"ensemble_p50": float(np.percentile(ens_ms, 50)),
                        "ensemble_p99": float(np.percentile(ens_ms, 99)),
                        "plain_p50": float(np.percentile(plain_lat, 50)),
@@ -484,9 +508,10 @@ def main():
                   "speedup_batched": value / plain_sps, "parity_first_scenarios": bool(parity),
                   "config1_ms": float(np.mean(c1_lat)), "config1_p50_ms": float(np.percentile(c1_lat, 50)),
                   "config1_p99_ms": float(np.percentile(c1_lat, 99))},
+        "autotune": tune_1024,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "propagate_persistent" if args.mode == 0 else "propagate_sweep",
+                     "kernel": "propagate_sweep" if args.mode == 1 else "propagate_persistent",
                      "kernel_ms": ens_relax_ms, "algorithmic_bytes": alg_bytes},
         "rothermel": {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,
                       "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS,

@@ -61,7 +61,8 @@ typedef struct pg_tuning {
   uint32_t tile_w, tile_h;      /* cells per tile (8 x 8; 0 = default)         */
   uint32_t workgroups;          /* persistent workgroups (0 = fill the device) */
   uint32_t max_iterations;      /* Jacobi sweep cap (0 = 64 * (W + H) + 1024)  */
-  uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep */
+  uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep,
+                                   2 = persistent kernel through a cached hipGraph */
   uint32_t spin_limit_log2;     /* device wait limit, 2^n polls (0 = 22)       */
   uint32_t waves_per_group;     /* persistent kernel waves per workgroup (0 = auto) */
   uint32_t profile;             /* 1: time the sweep phases (fills pg_stats.phase_ms) */

@@ -339,6 +339,7 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   sc.st[1] = sb.st1;
   sc.stamp = sb.stamp;
   sc.box = sb.box;
+  sc.blk = sb.blk;
   const uint32_t nwords = 1u << sc.nwords_log2;
   sc.mark[0] = reinterpret_cast<uint32_t *>(b + L.mark);
   sc.mark[1] = sc.mark[0] + nwords;
@@ -460,8 +461,7 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   L.cache = 1;
   if (live == 1 && nslots == 1) {
     L.lds_bits_src = c->scen[only].bits;
-    const size_t n = static_cast<size_t>(c->scen[only].sc.g.W) * c->scen[only].sc.g.H;
-    L.lds_bits_words = (n + 31) / 32;
+    L.lds_bits_words = (static_cast<size_t>(c->scen[only].sc.g.W) * c->scen[only].sc.g.H + 31) / 32;
   }
   pg_stats local{};
   LaunchResult res{};

@@ -56,10 +56,6 @@ namespace {
 
 constexpr uint32_t kNoRef = 0xFFFFFFFFu;
 constexpr int kStampNever = INT_MIN / 2;
-// DDA cells tested per batch: the bit loads of a batch are in flight
-// together; global (L2) loads need more of them than LDS loads.
-template <bool kLdsBits>
-constexpr int walk_batch() { return kLdsBits ? 8 : 16; }
 constexpr int kWave = 64;               // lanes per wavefront
 
 // ------------------------------------------------------------ control block
@@ -228,12 +224,35 @@ __device__ inline uint32_t make_ref(int x, int y) {
 
 // Dynamic LDS of the persistent kernel: per-wave tile slices, then the
 // boundary bitset when it fits.
-extern __shared__ uint32_t dyn_lds[];
+extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
+
+// Where a walk reads the boundary bits.  From global memory (L2) it reads a
+// copy of the bitset in 8x8 blocks: bit (x, y) is bit (x & 7) | (y & 7) << 3
+// of the 64-bit word of block (x >> 3, y >> 3), blocks row-major (bw per
+// row), so eight consecutive DDA cells lie in at most 2 x 2 blocks, loaded
+// together once per batch of eight instead of one dependent load per cell.
+// From LDS (the whole bitset staged there) it reads the row-major bitset,
+// one 32-bit word per cell visited, which LDS latency makes cheaper than
+// selecting among blocks.
+struct BitView {
+  const unsigned long long *blk;  // blocked copy (global form)
+  unsigned lds_off;               // row-major bitset in dyn_lds (LDS form)
+  int W, bw, bh;
+};
 
 template <bool kLdsBits>
-__device__ inline bool bit_at(const uint32_t *gbits, unsigned lds_off, uint32_t idx) {
-  const uint32_t w = kLdsBits ? dyn_lds[lds_off + (idx >> 5)] : G(gbits)[idx >> 5];
-  return (w >> (idx & 31u)) & 1u;
+__device__ inline unsigned long long blk_at(const BitView &bv, uint32_t b) {
+  return G(bv.blk)[b];
+}
+
+template <bool kLdsBits>
+__device__ inline bool bit_at(const BitView &bv, int x, int y) {
+  if (kLdsBits) {
+    const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(y) * static_cast<uint32_t>(bv.W);
+    return (dyn_lds[bv.lds_off + (i >> 5)] >> (i & 31u)) & 1u;
+  }
+  const unsigned long long w = G(bv.blk)[static_cast<uint32_t>((x >> 3) + (y >> 3) * bv.bw)];
+  return (w >> ((x & 7) | ((y & 7) << 3))) & 1ull;
 }
 
 // (double)d < 1e-9 for a float d >= 0 is exactly d <= this float (the float
@@ -244,74 +263,109 @@ constexpr float kTie = 0x1.12e0bep-30f;
 // (fx,fy) to (tx,ty), both included.  ru, rv count the steps left along each
 // axis: the reference's "cur is still between from and to" (propag.h:117-120)
 // is ru >= 0 && rv >= 0, i.e. (ru | rv) >= 0, and once the walk has visited
-// `to` or stepped past it one of them is negative for good.  With
-// d = tmax_x - tmax_y (exact whenever |d| is near the tie threshold, the two
-// being within a factor of two of each other), the reference's branches
-// become: x advances iff d <= kTie (tie or tmax_x < tmax_y), y advances iff
-// not d < -kTie (tie or tmax_x >= tmax_y; also for from == to, where d is
-// NaN, so that walk visits its one cell and ends).  Cells are tested
-// kWalkBatch at a time so a batch's bit loads are in flight together;
-// `steps` counts the cells up to and including the first boundary, as a
-// serial walk would.
+// `to` or stepped past it one of them is negative for good; the cell visited
+// with ru, rv left is (tx - sx*ru, ty - sy*rv).  With d = tmax_x - tmax_y
+// (exact whenever |d| is near the tie threshold, the two being within a
+// factor of two of each other), the reference's branches become: x advances
+// iff d <= kTie (tie or tmax_x < tmax_y), y advances iff not d < -kTie (tie
+// or tmax_x >= tmax_y; also for from == to, where d is NaN, so that walk
+// visits its one cell and ends).  Cells are tested eight at a time; `steps`
+// counts the cells up to and including the first boundary, as a serial
+// walk would.
 template <bool kLdsBits>
-__device__ bool walk_blocked(int fx, int fy, int tx, int ty, const uint32_t *gbits, unsigned lds_off,
-                             int W, uint32_t &steps) {
+__device__ bool walk_blocked(int fx, int fy, int tx, int ty, const BitView &bv, uint32_t &steps) {
   const int ddx = tx - fx, ddy = ty - fy;
   int ru = abs(ddx), rv = abs(ddy);
-  const int ix = (ddx > 0) - (ddx < 0);
-  const int iy = ((ddy > 0) - (ddy < 0)) * W;
+  const int sx = (ddx > 0) - (ddx < 0), sy = (ddy > 0) - (ddy < 0);
   // 1.0 / float(abs(..)) is a double division narrowed to float
   const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ru)));
   const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(rv)));
   float tmx = dlx, tmy = dly;
-  int idx = fx + fy * W;
-  for (;;) {
-    constexpr int kWalkBatch = walk_batch<kLdsBits>();
-    uint32_t id[kWalkBatch];
-    int live[kWalkBatch];
+  if constexpr (kLdsBits) {
+    const int iy = sy * bv.W;
+    int idx = fx + fy * bv.W;
+    for (;;) {
+      uint32_t id[8];
+      int live[8];
 #pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b) {
-      live[b] = ru | rv;
-      id[b] = live[b] >= 0 ? static_cast<uint32_t>(idx) : 0u;
-      const float d = tmx - tmy;
-      const bool xs = d <= kTie, ys = !(d < -kTie);
-      ru -= xs ? 1 : 0;
-      rv -= ys ? 1 : 0;
-      idx += (xs ? ix : 0) + (ys ? iy : 0);
-      tmx = xs ? tmx + dlx : tmx;
-      tmy = ys ? tmy + dly : tmy;
-    }
-    uint32_t hits = 0;
+      for (int b = 0; b < 8; ++b) {
+        live[b] = ru | rv;
+        id[b] = live[b] >= 0 ? static_cast<uint32_t>(idx) : 0u;
+        const float d = tmx - tmy;
+        const bool xs = d <= kTie, ys = !(d < -kTie);
+        ru -= xs ? 1 : 0;
+        rv -= ys ? 1 : 0;
+        idx += (xs ? sx : 0) + (ys ? iy : 0);
+        tmx = xs ? tmx + dlx : tmx;
+        tmy = ys ? tmy + dly : tmy;
+      }
+      uint32_t hits = 0;
 #pragma unroll
-    for (int b = 0; b < kWalkBatch; ++b)
-      hits |= (bit_at<kLdsBits>(gbits, lds_off, id[b]) && live[b] >= 0) ? (1u << b) : 0u;
-    if (hits) {
-      steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
-      return true;
-    }
-    static_assert(kWalkBatch <= 32, "hits is a 32-bit mask");
-    if ((ru | rv) < 0) {
+      for (int b = 0; b < 8; ++b)
+        hits |= (((dyn_lds[bv.lds_off + (id[b] >> 5)] >> (id[b] & 31u)) & 1u) && live[b] >= 0) ? (1u << b) : 0u;
+      if (hits) {
+        steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
+        return true;
+      }
+      if ((ru | rv) < 0) {
 #pragma unroll
-      for (int b = 0; b < kWalkBatch; ++b) steps += live[b] >= 0 ? 1u : 0u;
-      return false;
+        for (int b = 0; b < 8; ++b) steps += live[b] >= 0 ? 1u : 0u;
+        return false;
+      }
+      steps += 8;
     }
-    steps += kWalkBatch;
+  } else {
+    for (;;) {
+      // The batch's cells lie between (x0, y0) and (x0 + 7 sx, y0 + 7 sy).
+      const int x0 = tx - sx * ru, y0 = ty - sy * rv;
+      const int xl = max(min(x0, x0 + 7 * sx), 0) >> 3, yl = max(min(y0, y0 + 7 * sy), 0) >> 3;
+      const int xh = min(xl + 1, bv.bw - 1), yh = min(yl + 1, bv.bh - 1);
+      const unsigned long long w00 = blk_at<false>(bv, static_cast<uint32_t>(xl + yl * bv.bw));
+      const unsigned long long w10 = blk_at<false>(bv, static_cast<uint32_t>(xh + yl * bv.bw));
+      const unsigned long long w01 = blk_at<false>(bv, static_cast<uint32_t>(xl + yh * bv.bw));
+      const unsigned long long w11 = blk_at<false>(bv, static_cast<uint32_t>(xh + yh * bv.bw));
+      int lx = x0 - (xl << 3), ly = y0 - (yl << 3);  // in [0, 16) while live
+      uint32_t hits = 0, live_n = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool live = (ru | rv) >= 0;
+        const unsigned long long w = (ly & 8) ? ((lx & 8) ? w11 : w01) : ((lx & 8) ? w10 : w00);
+        const bool bit = (w >> ((lx & 7) | ((ly & 7) << 3))) & 1ull;
+        hits |= (live && bit) ? (1u << b) : 0u;
+        live_n += live ? 1u : 0u;
+        const float d = tmx - tmy;
+        const bool xs = d <= kTie, ys = !(d < -kTie);
+        ru -= xs ? 1 : 0;
+        rv -= ys ? 1 : 0;
+        lx += xs ? sx : 0;
+        ly += ys ? sy : 0;
+        tmx = xs ? tmx + dlx : tmx;
+        tmy = ys ? tmy + dly : tmy;
+      }
+      if (hits) {
+        steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
+        return true;
+      }
+      steps += live_n;
+      if ((ru | rv) < 0) return false;
+    }
   }
 }
 
 // Line-of-sight flag for a committed reference: the DDA from the cell to an
 // adjacent reference visits exactly those two cells.
 template <bool kLdsBits>
-__device__ inline bool line_clear(int x, int y, int rx, int ry, const uint32_t *gbits, unsigned lds_off,
-                                  int W, uint32_t &steps) {
+__device__ inline bool line_clear(int x, int y, int rx, int ry, const BitView &bv, uint32_t &steps) {
   if (rx == x && ry == y) return false;  // an ignition's own reference: never walked
   if (abs(rx - x) <= 1 && abs(ry - y) <= 1) {
     steps += 2;
-    const uint32_t a = static_cast<uint32_t>(x) + static_cast<uint32_t>(y) * static_cast<uint32_t>(W);
-    const uint32_t b = static_cast<uint32_t>(rx) + static_cast<uint32_t>(ry) * static_cast<uint32_t>(W);
-    return !bit_at<kLdsBits>(gbits, lds_off, a) && !bit_at<kLdsBits>(gbits, lds_off, b);
+    return !bit_at<kLdsBits>(bv, x, y) && !bit_at<kLdsBits>(bv, rx, ry);
   }
-  return !walk_blocked<kLdsBits>(x, y, rx, ry, gbits, lds_off, W, steps);
+  return !walk_blocked<kLdsBits>(x, y, rx, ry, bv, steps);
+}
+
+__device__ inline BitView scen_bits(const Scen &sc, unsigned lds_off) {
+  return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.ntx, sc.nty};
 }
 
 __device__ inline uint2 pack_state(float t, uint32_t ref) { return make_uint2(__float_as_uint(t), ref); }
@@ -329,7 +383,7 @@ struct Args {
   Slot *slots;               // grid-barrier arrival lines, one per workgroup
   unsigned spin_limit;
   int lds_bits;              // bitset of scenario 0 staged in LDS
-  const uint32_t *lds_src;   // that bitset
+  const uint32_t *lds_src;   // that bitset (row-major)
   unsigned lds_words;
   unsigned long long *prof;  // [sweep - k_begin][kProfCols] profile ticks, or null
   unsigned prof_sweeps;      // rows of prof
@@ -412,7 +466,7 @@ __device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) 
     const Scen &sc = a.scen[e.si];
     const int x = static_cast<int>(e.cell & 0xffffu), y = static_cast<int>(e.cell >> 16);
     uint32_t ref = e.ref;
-    if (line_clear<false>(x, y, ref_x<true>(ref), ref_y<true>(ref), sc.bits, 0, sc.g.W, cnt.steps)) ref |= kClearBit;
+    if (line_clear<false>(x, y, ref_x<true>(ref), ref_y<true>(ref), scen_bits(sc, 0), cnt.steps)) ref |= kClearBit;
     uint2 *st_out = (k & 1) ? sc.st[0] : sc.st[1];
     st_state(&st_out[static_cast<size_t>(x) + static_cast<size_t>(y) * sc.g.W], pack_state(e.time, ref));
   }
@@ -624,7 +678,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   typedef float f4v __attribute__((ext_vector_type(4)));
   const PG_GLOBAL f4v *__restrict__ coef = G(reinterpret_cast<const f4v *>(sc.coef));
   int *__restrict__ stamp = sc.stamp;
-  const uint32_t *__restrict__ bits = sc.bits;
+  const BitView bv = scen_bits(sc, lds_off);
   // (selects, not sc.st[k & 1]: a dynamic index would put sc in scratch)
   const uint2 *__restrict__ st_in = (k & 1) ? sc.st[1] : sc.st[0];
   uint2 *__restrict__ st_out = (k & 1) ? sc.st[0] : sc.st[1];
@@ -734,7 +788,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
           if (kCache)
             blocked = (ref & kClearBit) == 0u;
           else
-            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, bits, lds_off, g.W, cnt.steps);
+            blocked = walk_blocked<kLdsBits>(nx, ny, rfx, rfy, bv, cnt.steps);
           const uint32_t cand_ref = blocked ? make_ref(nx, ny) : make_ref(rfx, rfy);
           const float cand_t = blocked ? nt : ct;
           if (cand_t < best_t) {
@@ -776,8 +830,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
           best_ref = old;
         else if (kDefer)
           deferred = true;
-        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), bits, lds_off, g.W,
-                                      cnt.steps))
+        else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), bv, cnt.steps))
           best_ref |= kClearBit;
       }
       if (a.prof) walk_ticks += wall_clock64() - tw;
@@ -1240,7 +1293,7 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
     }
     if (burned && sc.cache) {
       uint32_t steps = 0;
-      if (line_clear<false>(x, y, rx[i], ry[i], sc.bits, 0, g.W, steps)) ref |= kClearBit;
+      if (line_clear<false>(x, y, rx[i], ry[i], scen_bits(sc, 0), steps)) ref |= kClearBit;
     }
     sc.st[0][i] = pack_state(t, ref);
     sc.st[1][i] = pack_state(t, ref);
@@ -1265,6 +1318,23 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
 // pre_propagate's find_boundaries (propag.cu:345-375): a cell is a boundary
 // if an in-bounds neighbour has a dissimilar fire that is faster.  One wave
 // covers 64 consecutive cells = two bitset words, so no atomics are needed.
+// The boundary bitset in 8x8 blocks (the walks' layout, see blk_at).
+__global__ void block_bits(const uint32_t *bits, unsigned long long *blk, uint32_t W, uint32_t H, uint32_t bw,
+                           uint32_t bh) {
+  const size_t nb = static_cast<size_t>(bw) * bh;
+  for (size_t b = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < nb;
+       b += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const uint32_t x0 = static_cast<uint32_t>(b % bw) * 8u, y0 = static_cast<uint32_t>(b / bw) * 8u;
+    unsigned long long v = 0;
+    for (uint32_t r = 0; r < 8u && y0 + r < H; ++r)
+      for (uint32_t c = 0; c < 8u && x0 + c < W; ++c) {
+        const size_t i = static_cast<size_t>(x0 + c) + static_cast<size_t>(y0 + r) * W;
+        if ((bits[i >> 5] >> (i & 31u)) & 1u) v |= 1ull << (c | (r << 3));
+      }
+    blk[b] = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void boundaries_kernel(Geom g, const float *speed, const float *az,
                                                         const float *ecc, uint32_t *bits) {
   const size_t n = static_cast<size_t>(g.W) * g.H;
@@ -1363,11 +1433,30 @@ pg_status ensure(DeviceBuffer *buf, size_t bytes) {
   return PG_OK;
 }
 
+// An instantiated launch graph of the persistent kernel: clear the control
+// block and barrier slots, record the start event, run the kernel
+// (cooperative node), record the end event, copy the control block to
+// pinned host memory.  Built once per kernel variant and launch shape and
+// re-parameterised per call.
+struct LaunchGraph {
+  void (*kern)(Args) = nullptr;
+  unsigned groups = 0, block = 0;
+  size_t shmem = 0;
+  void *base = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t knode = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Control *host = nullptr;
+};
+
 // Per device: the buffers of the one-scenario entry point and of a launch
-// (control block, barrier slots, profile rows), each behind its own lock.
+// (control block, barrier slots, profile rows), each behind its own lock,
+// and the launch graphs (under the launch lock).
 struct PerDevice {
   std::mutex call_mu, launch_mu;
   DeviceBuffer call, launch;
+  std::vector<LaunchGraph *> graphs;
 };
 std::mutex g_dev_mu;
 std::vector<PerDevice *> g_dev;
@@ -1380,6 +1469,82 @@ PerDevice *per_device(int device) {
 }
 
 constexpr unsigned kProfRows = 8192;
+
+pg_status build_graph(LaunchGraph *lg, const Args &a, size_t max_groups) {
+  PG_HIP(hipGraphCreate(&lg->graph, 0));
+  PG_HIP(hipEventCreate(&lg->ev0));
+  PG_HIP(hipEventCreate(&lg->ev1));
+  PG_HIP(hipHostMalloc(reinterpret_cast<void **>(&lg->host), sizeof(Control), hipHostMallocDefault));
+  uint32_t *ctl = reinterpret_cast<uint32_t *>(a.ctl);
+  uint32_t *slots = reinterpret_cast<uint32_t *>(a.slots);
+  size_t ctl_words = sizeof(Control) / 4, slot_words = max_groups * sizeof(Slot) / 4;
+  uint32_t zero = 0;
+  void *args_ctl[] = {&ctl, &ctl_words, &zero};
+  void *args_slots[] = {&slots, &slot_words, &zero};
+  hipKernelNodeParams p{};
+  p.func = reinterpret_cast<void *>(fill_u32);
+  p.blockDim = dim3(256);
+  p.gridDim = dim3(1);
+  p.kernelParams = args_ctl;
+  hipGraphNode_t n_ctl, n_slots, n_ev0, n_ev1, n_copy;
+  PG_HIP(hipGraphAddKernelNode(&n_ctl, lg->graph, nullptr, 0, &p));
+  p.gridDim = dim3(static_cast<unsigned>((slot_words + 255) / 256));
+  p.kernelParams = args_slots;
+  PG_HIP(hipGraphAddKernelNode(&n_slots, lg->graph, nullptr, 0, &p));
+  hipGraphNode_t fills[2] = {n_ctl, n_slots};
+  PG_HIP(hipGraphAddEventRecordNode(&n_ev0, lg->graph, fills, 2, lg->ev0));
+  Args ak = a;
+  void *args_k[] = {&ak};
+  hipKernelNodeParams pk{};
+  pk.func = reinterpret_cast<void *>(lg->kern);
+  pk.gridDim = dim3(lg->groups);
+  pk.blockDim = dim3(lg->block);
+  pk.sharedMemBytes = static_cast<unsigned>(lg->shmem);
+  pk.kernelParams = args_k;
+  PG_HIP(hipGraphAddKernelNode(&lg->knode, lg->graph, &n_ev0, 1, &pk));
+  hipKernelNodeAttrValue coop{};
+  coop.cooperative = 1;
+  PG_HIP(hipGraphKernelNodeSetAttribute(lg->knode, hipKernelNodeAttributeCooperative, &coop));
+  PG_HIP(hipGraphAddEventRecordNode(&n_ev1, lg->graph, &lg->knode, 1, lg->ev1));
+  PG_HIP(hipGraphAddMemcpyNode1D(&n_copy, lg->graph, &n_ev1, 1, lg->host, a.ctl, sizeof(Control),
+                                 hipMemcpyDeviceToHost));
+  PG_HIP(hipGraphInstantiate(&lg->exec, lg->graph, nullptr, nullptr, 0));
+  return PG_OK;
+}
+
+void free_graph(LaunchGraph *lg) {
+  if (lg->exec) (void)hipGraphExecDestroy(lg->exec);
+  if (lg->graph) (void)hipGraphDestroy(lg->graph);
+  if (lg->ev0) (void)hipEventDestroy(lg->ev0);
+  if (lg->ev1) (void)hipEventDestroy(lg->ev1);
+  if (lg->host) (void)hipHostFree(lg->host);
+  delete lg;
+}
+
+// The cached graph for this launch shape (built on first use).
+pg_status launch_graph_for(PerDevice *pd, void (*kern)(Args), unsigned groups, unsigned block, size_t shmem,
+                           const Args &a, size_t max_groups, LaunchGraph **out) {
+  for (LaunchGraph *lg : pd->graphs)
+    if (lg->kern == kern && lg->groups == groups && lg->block == block && lg->shmem == shmem &&
+        lg->base == pd->launch.base) {
+      *out = lg;
+      return PG_OK;
+    }
+  auto *lg = new LaunchGraph();
+  lg->kern = kern;
+  lg->groups = groups;
+  lg->block = block;
+  lg->shmem = shmem;
+  lg->base = pd->launch.base;
+  pg_status st = build_graph(lg, a, max_groups);
+  if (st != PG_OK) {
+    free_graph(lg);
+    return st;
+  }
+  pd->graphs.push_back(lg);
+  *out = lg;
+  return PG_OK;
+}
 
 }  // namespace
 
@@ -1394,6 +1559,7 @@ size_t scen_buffer_bytes(size_t W, size_t H) {
   off = align_up(off + n * sizeof(uint2), 256);
   off = align_up(off + n * sizeof(int), 256);
   off = align_up(off + ntiles * sizeof(uint4), 256);
+  off = align_up(off + ntiles * sizeof(unsigned long long), 256);
   return off;
 }
 
@@ -1406,7 +1572,9 @@ ScenBuffers scen_carve(void *base, size_t W, size_t H) {
   b.st0 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.st1 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.stamp = reinterpret_cast<int *>(p + off); off = align_up(off + n * sizeof(int), 256);
-  b.box = reinterpret_cast<uint4 *>(p + off);
+  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+  b.box = reinterpret_cast<uint4 *>(p + off); off = align_up(off + ntiles * sizeof(uint4), 256);
+  b.blk = reinterpret_cast<unsigned long long *>(p + off);
   return b;
 }
 
@@ -1455,6 +1623,9 @@ pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenO
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, err, static_cast<size_t>(1), 0u);
   hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, sc.g, speed, az, ecc,
                      boundaries);
+  hipLaunchKernelGGL(block_bits, grid_for(ntiles), dim3(256), 0, stream, boundaries,
+                     const_cast<unsigned long long *>(sc.blk), sc.g.W, sc.g.H, static_cast<uint32_t>(sc.ntx),
+                     static_cast<uint32_t>(sc.nty));
   hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, sc, si, q, err, speed, az, ecc, time, refs_x,
                      refs_y);
   PG_HIP(hipGetLastError());
@@ -1503,6 +1674,10 @@ pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *
 
 // ---------------------------------------------------------------- launches
 
+static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const unsigned long long *counters,
+                              float rms, unsigned launched_groups, const Args &a, unsigned prof_rows,
+                              pg_stats *stats, LaunchResult *res);
+
 pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tuning, pg_stats *stats,
                      LaunchResult *res, hipStream_t stream) {
   pg_tuning tn{};
@@ -1542,10 +1717,15 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (b > cu * 8) b = cu * 8;
     return dim3(static_cast<unsigned>(b ? b : 1));
   };
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
-                     sizeof(Control) / 4, 0u);
-  hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t *>(a.slots), max_groups * sizeof(Slot) / 4, 0u);
+  // mode 2: the persistent launch through a cached hipGraph (no profiling)
+  const bool use_graph = tn.mode == 2 && !prof_rows;
+  if (tn.mode > 2) return set_error(PG_ERR_INVALID, "mode must be 0, 1 or 2");
+  if (!use_graph) {
+    hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
+                       sizeof(Control) / 4, 0u);
+    hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t *>(a.slots), max_groups * sizeof(Slot) / 4, 0u);
+  }
   if (prof_rows)
     hipLaunchKernelGGL(fill_u32, grid_for(static_cast<size_t>(prof_rows) * kProfCols * 2), dim3(256), 0, stream,
                        reinterpret_cast<uint32_t *>(a.prof), static_cast<size_t>(prof_rows) * kProfCols * 2, 0u);
@@ -1597,7 +1777,10 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     // (line-of-sight flags cached and the bitset read from L2)
     const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(WalkReq) * kWalkQ : 0);
     const int max_waves = 16;  // the kernel's launch bound: 1024 threads
-    if (tn.waves_per_group) waves = static_cast<int>(tn.waves_per_group);
+    if (tn.waves_per_group) {
+      // with the bitset in LDS, at most the slices that fit beside it
+      waves = a.lds_bits ? std::min(waves, static_cast<int>(tn.waves_per_group)) : static_cast<int>(tn.waves_per_group);
+    }
     if (waves > max_waves) waves = max_waves;
     const size_t shmem = waves * wave_lds + (a.lds_bits ? bits_bytes : 0);
     const int block = waves * kWave;
@@ -1611,6 +1794,27 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (groups > 64u * kWave) groups = 64u * kWave;  // the barrier master polls <= 64 slots per lane
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
     launched_groups = groups;
+    if (use_graph) {
+      LaunchGraph *lg = nullptr;
+      st = launch_graph_for(pd, kern, groups, static_cast<unsigned>(block), shmem, a, max_groups, &lg);
+      if (st != PG_OK) return st;
+      hipKernelNodeParams pk{};
+      void *kargs[] = {&a};
+      pk.func = reinterpret_cast<void *>(kern);
+      pk.gridDim = dim3(groups);
+      pk.blockDim = dim3(static_cast<unsigned>(block));
+      pk.sharedMemBytes = static_cast<unsigned>(shmem);
+      pk.kernelParams = kargs;
+      PG_HIP(hipGraphExecKernelNodeSetParams(lg->exec, lg->knode, &pk));
+      PG_HIP(hipGraphLaunch(lg->exec, stream));
+      PG_HIP(hipStreamSynchronize(stream));
+      std::memcpy(book, &lg->host->book, sizeof(book));
+      unsigned long long counters[12];
+      std::memcpy(counters, lg->host->stats, sizeof(counters));
+      float gms = 0.0f;
+      PG_HIP(hipEventElapsedTime(&gms, lg->ev0, lg->ev1));
+      return finish_stats(dev, book, counters, gms, launched_groups, a, prof_rows, stats, res);
+    }
     void *kargs[] = {&a};
     PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups), dim3(block), kargs,
                                       static_cast<unsigned>(shmem), stream));
@@ -1618,12 +1822,19 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
   }
-  if (book[3] & 1u) return set_error(PG_ERR_TIMEOUT, "a grid barrier wait exceeded its spin limit");
   unsigned long long counters[12];
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
   float rms = 0.0f;
   PG_HIP(hipEventElapsedTime(&rms, evr0, evr1));
+  return finish_stats(dev, book, counters, rms, launched_groups, a, prof_rows, stats, res);
+}
+
+// Launch outcome and counters into res / stats.
+static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const unsigned long long *counters, float rms,
+                       unsigned launched_groups, const Args &a, unsigned prof_rows, pg_stats *stats,
+                       LaunchResult *res) {
+  if (book[3] & 1u) return set_error(PG_ERR_TIMEOUT, "a grid barrier wait exceeded its spin limit");
   res->last_sweep = static_cast<int>(book[4]);
   res->finished = book[1] ? 1 : 0;
   if (stats) {
@@ -1695,6 +1906,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   sc.st[1] = b.st1;
   sc.stamp = b.stamp;
   sc.box = b.box;
+  sc.blk = b.blk;
   sc.bits = boundaries;
   sc.k0 = 0;
   TileQueue q;

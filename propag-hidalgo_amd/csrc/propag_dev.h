@@ -34,10 +34,11 @@ struct Scen {
   const Coef *coef;
   uint2 *st[2];          // double-buffered (time bits, reference word)
   int *stamp;            // last sweep in which the cell changed
-  const uint32_t *bits;  // boundary bitset
+  const uint32_t *bits;  // boundary bitset (row-major, the output layout)
+  const unsigned long long *blk;  // the same bits in 8x8 blocks (walks)
   uint4 *box;            // per tile: tile box (x0, y0, x1, y1) of far dependents
   uint32_t *mark[2];     // tiles queued for a sweep of parity 0 / 1
-  int ntx, nty;
+  int ntx, nty;          // tiles (= 8x8 bit blocks) per row / column
   uint32_t nwords_log2, pad0;
   int k0;                // sweep number of the scenario's first sweep
   unsigned max_iters;    // sweeps allowed from k0
@@ -79,6 +80,7 @@ struct ScenBuffers {
   uint2 *st0, *st1;
   int *stamp;
   uint4 *box;
+  unsigned long long *blk;  // blocked boundary bitset
 };
 size_t scen_buffer_bytes(size_t width, size_t height);
 ScenBuffers scen_carve(void *base, size_t width, size_t height);
@@ -123,7 +125,7 @@ struct Launch {
   TileQueue q;
   int k_begin, k_end;
   int cache;           // all scenarios cache line-of-sight flags
-  const uint32_t *lds_bits_src;  // nscen == 1: the bitset, staged in LDS if it fits
+  const uint32_t *lds_bits_src;  // nscen == 1: the (row-major) bitset, staged in LDS if it fits
   size_t lds_bits_words;
 };
 

@@ -296,7 +296,10 @@ def _tuning(tuning):
     if isinstance(tuning, Tuning):
         return tuning
     t = Tuning()
+    names = {f[0] for f in Tuning._fields_}
     for k, v in dict(tuning).items():
+        if k not in names:
+            raise PropagValidationError(f"unknown tuning field '{k}'")
         setattr(t, k, int(v))
     return t
 

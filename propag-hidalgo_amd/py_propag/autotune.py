@@ -1,0 +1,99 @@
+"""Autotuner of the batched propagation launch.
+
+It picks the persistent kernel's super-grid — workgroups per launch and
+waves per workgroup, i.e. how many 8x8 tiles are in flight per CU and how
+the per-sweep tile queue is split — and the launch form (direct cooperative
+launch or the cached hipGraph), by timing candidates on a sample of the
+ensemble to be run.  The sweep schedule is exact Jacobi, so results never
+depend on the tuning and any sample is a fair trial; the tuner checks that
+every trial finishes every scenario in the same number of sweeps.
+
+Two shapes are fixed rather than tuned: the tile (8x8 cells, one per lane
+of a 64-lane wave) and one sweep per grid barrier (more local sweeps
+between barriers would change the arrival times the exact schedule
+defines).
+
+The reference sizes its super-grid from the occupancy of its cooperative
+kernel (crates/propag/src/propag.rs:403-418) and has no tuner.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+
+def default_space():
+    """Candidate tunings: waves per workgroup x fraction of the workgroups
+    that fit x launch form."""
+    out = []
+    for waves, frac, mode in itertools.product((16, 8, 4), (1.0, 0.5), (0, 2)):
+        out.append({"waves_per_group": waves, "group_fraction": frac, "mode": mode})
+    return out
+
+
+class Autotuner:
+    """Times candidate tunings of ``Context.run`` on sample ensembles and
+    remembers the best per workload key."""
+
+    def __init__(self, ctx, space=None, repeats: int = 1):
+        self.ctx = ctx
+        self.space = space if space is not None else default_space()
+        self.repeats = repeats
+        self.best = {}     # key -> tuning dict
+        self.tables = {}   # key -> [(tuning, seconds)]
+
+    @staticmethod
+    def key(width: int, height: int, count: int):
+        """Workload class: raster size and ensemble size, each rounded to a
+        power of two."""
+        def p2(v):
+            return 1 << max(0, (int(v) - 1).bit_length())
+        return (p2(width), p2(height), p2(count))
+
+    def _trial(self, terrain_id, cases, tuning):
+        ids = [self.ctx.admit(terrain_id, c["wx"], c["ign"], c["max_time"]) for c in cases]
+        if any(i is None for i in ids):
+            for i in ids:
+                if i is not None:
+                    self.ctx.release(i)
+            raise RuntimeError("autotune sample does not fit the memory budget")
+        t0 = time.perf_counter()
+        fin, stats = self.ctx.run(tuning=tuning)
+        dt = time.perf_counter() - t0
+        sweeps = [self.ctx.status(i)[1] for i in ids]
+        for i in ids:
+            self.ctx.release(i)
+        if fin != len(ids):
+            raise RuntimeError("autotune trial left scenarios unfinished")
+        return dt, sweeps, stats
+
+    def tune(self, terrain_id: int, cases, width: int, height: int, count: int | None = None):
+        """Runs every candidate on ``cases`` (a sample of the ensemble) and
+        returns the fastest tuning (a dict for Context.run)."""
+        k = self.key(width, height, count if count is not None else len(cases))
+        if k in self.best:
+            return self.best[k]
+        # reference sweep counts from an untuned run; the device's full
+        # workgroup count for each waves-per-group choice is what a launch
+        # with workgroups = 0 reports
+        _, ref_sweeps, _ = self._trial(terrain_id, cases, None)
+        full = {}
+        table = []
+        for cand in sorted(self.space, key=lambda c: -c["group_fraction"]):
+            w = cand["waves_per_group"]
+            if cand["group_fraction"] < 1.0 and w not in full:
+                continue  # needs the full count of this shape first
+            groups = 0 if cand["group_fraction"] >= 1.0 else max(1, int(full[w] * cand["group_fraction"]))
+            tuning = {"waves_per_group": w, "mode": cand["mode"], "workgroups": groups}
+            best_dt = None
+            for _ in range(self.repeats):
+                dt, sweeps, st = self._trial(terrain_id, cases, tuning)
+                if sweeps != ref_sweeps:
+                    raise RuntimeError(f"tuning {tuning} changed the sweep counts")
+                best_dt = dt if best_dt is None else min(best_dt, dt)
+            full.setdefault(w, st["workgroups"])
+            table.append(({**tuning, "workgroups": st["workgroups"]}, best_dt))
+        table.sort(key=lambda e: e[1])
+        self.tables[k] = table
+        self.best[k] = table[0][0]
+        return table[0][0]

@@ -273,3 +273,53 @@ def test_server_stream_each_result_same_as_alone():
         w, h = shapes[c["terrain"]]
         _same(got[i], _alone(pp, lands[c["terrain"]], w, h, c["wx"], c["ign"], c["max_time"]))
     print(f"served {len(cases)} requests in {server.launches} launches, {server.busy_refusals} refusals")
+
+
+def test_graph_launches_match_direct_launches():
+    """mode 2 (the persistent launch through a cached, re-parameterised
+    hipGraph), replayed over many short launches with host work between
+    them, gives the same results as direct launches."""
+    import py_propag as pp
+
+    w, h = 140, 110
+    land = _terrain(90, w, h)
+    cases = [_case(s, w, h) for s in range(90, 95)]
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        out = {}
+        for mode in (0, 2):
+            ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+            launches = 0
+            while any(ctx.status(i)[0] == pp.RUNNING for i in ids):
+                ctx.run(max_sweeps=7, tuning={"mode": mode})
+                launches += 1
+                assert launches < 2000
+            out[mode] = [ctx.result(i) for i in ids]
+            for i in ids:
+                ctx.release(i)
+        assert launches > 3
+        for a, b in zip(out[0], out[2]):
+            _same(a, type("R", (), {"arrival_times": b[0], "refs_x": b[1], "refs_y": b[2],
+                                    "boundary_change": b[3]})())
+        c = cases[0]
+        _same(out[2][0], _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+
+
+def test_autotuner_picks_a_tuning_without_changing_results():
+    import py_propag as pp
+    from py_propag.autotune import Autotuner
+
+    w, h = 128, 96
+    land = _terrain(91, w, h)
+    cases = [_case(s, w, h) for s in range(100, 106)]
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        tuner = Autotuner(ctx)
+        best = tuner.tune(tid, cases, w, h)
+        assert set(best) == {"waves_per_group", "workgroups", "mode"}
+        assert len(tuner.tables[tuner.key(w, h, len(cases))]) == len(tuner.space)
+        assert tuner.tune(tid, cases, w, h) is best   # cached per workload class
+        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+        ctx.run(tuning=best)
+        for sid, c in zip(ids, cases):
+            _same(ctx.result(sid), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
