@@ -72,6 +72,9 @@ def autotuned(args, ctx, tid, cases, W, H, tuning):
     """The ensemble's launch tuning: the autotuner's pick on a sample of
     --tune-sample scenarios (or the fixed bench tuning with --no-autotune /
     --profile-phases)."""
+    if args.tuning:
+        w, g, m = (int(v) for v in args.tuning.split(","))
+        return {**tuning, "waves_per_group": w, "workgroups": g, "mode": m}, None
     if args.no_autotune or args.profile_phases or args.mode == 1:
         return dict(tuning), None
     from py_propag.autotune import Autotuner
@@ -282,6 +285,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="run the ensembles with the default tuning")
     ap.add_argument("--tune-sample", type=int, default=32, help="scenarios per autotuner trial")
+    ap.add_argument("--tuning", default="", help="fixed ensemble tuning WAVES,GROUPS,MODE instead of the autotuner")
     ap.add_argument("--profile-phases", action="store_true",
                     help="time the sweep phases inside the kernel (slower; diagnostics only)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -422,7 +426,7 @@ def main():
     es = ens_stats[-1]
     log(f"batched: {np.mean(ens_ms):.1f} ms per ensemble of {args.batch} ({value:.1f} scenarios/s, "
         f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
-        f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits, {es['dda_steps']} walk steps; "
+        f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle), {es['dda_steps']} walk steps; "
         f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
         f"phase totals over waves {es['phase_total_ms']}, "
         f"barrier (start/span/gather/book) {es['barrier_ms']}")
@@ -521,7 +525,7 @@ def main():
         "large_4096": large,
         "cpu_baseline": cpu,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
-                                                       "tiles_visited", "tile_marks", "relax_ms", "elapsed_ms",
+                                                       "tiles_visited", "tile_marks", "idle_tiles", "relax_ms", "elapsed_ms",
                                                        "workgroups")},
                         "config1": {"sweeps": st1["iterations"], "relaxations": st1["relaxations"],
                                     "improvements": st1["improvements"], "dda_steps": st1["dda_steps"],

@@ -93,6 +93,7 @@ typedef struct pg_stats {
   uint64_t tile_marks;      /* tiles queued for a next sweep (with repeats)   */
   double   phase_total_ms[4]; /* with tuning.profile: the phases of phase_ms
                                  summed over all waves and sweeps (wave-ms)  */
+  uint64_t idle_tiles;      /* tile visits in which no cell needed the rule   */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

@@ -340,6 +340,7 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   sc.stamp = sb.stamp;
   sc.box = sb.box;
   sc.blk = sb.blk;
+  sc.isref = sb.isref;
   const uint32_t nwords = 1u << sc.nwords_log2;
   sc.mark[0] = reinterpret_cast<uint32_t *>(b + L.mark);
   sc.mark[1] = sc.mark[0] + nwords;

@@ -69,8 +69,9 @@ struct Control {
   Line gen;          // [0] barrier release: sweep + 1, bit 31 = finished
   Line book;         // [1] done, [3] error bits, [4] last sweep run
   Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
-  unsigned long long stats[12];  // relaxations, improvements, dda_steps, tiles, work, wait,
-                                 // critical-path ticks, marks, phase ticks summed over waves [4]
+  unsigned long long stats[16];  // relaxations, improvements, dda_steps, tiles, work, wait,
+                                 // critical-path ticks, marks, phase ticks summed over waves [4],
+                                 // tile visits without work
 };
 
 // Every pointer the kernels use points to device memory; pointers read from
@@ -94,6 +95,9 @@ __device__ inline unsigned add_relaxed(unsigned *p, unsigned v) {
 }
 __device__ inline uint32_t or_fetch(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_or(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void or_relaxed(uint32_t *p, uint32_t v) {
+  (void)__hip_atomic_fetch_or(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline void and_relaxed(uint32_t *p, uint32_t v) {
   (void)__hip_atomic_fetch_and(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -142,6 +146,14 @@ __device__ inline void st_state(uint2 *p, uint2 v) {
 // coherent ones: visible device-wide); the clobber keeps the compiler from
 // moving memory accesses across it.
 __device__ inline void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ inline void wave_sync() {
+  // Lanes of one wave exchange data through LDS: order the accesses.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ------------------------------------------------------------------ geometry
 
@@ -368,6 +380,7 @@ __device__ inline BitView scen_bits(const Scen &sc, unsigned lds_off) {
   return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.ntx, sc.nty};
 }
 
+
 __device__ inline uint2 pack_state(float t, uint32_t ref) { return make_uint2(__float_as_uint(t), ref); }
 
 
@@ -420,19 +433,12 @@ static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
 constexpr int kProfCols = 8;
 
 struct Counters {
-  uint32_t relax, improve, steps, tiles, marks;
+  uint32_t relax, improve, steps, tiles, marks, idle;
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
   uint64_t ph_total[4];             // profiling: this wave's phase ticks, all sweeps
 };
 
-__device__ inline void wave_sync() {
-  // Lanes of one wave exchange data through LDS: order the accesses.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Deferred line-of-sight walks.  When the boundary bitset is read from L2,
 // a tile's commit walks would keep only its few improving lanes busy for
@@ -548,11 +554,23 @@ __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
   return nt;
 }
 
+// imp: ballot of the tile's cells that changed (lane l = cell (l % 8, l / 8)).
+// A change of cell r in sweep k makes sweep k + 1 evaluate r's 3x3
+// neighbours and the 3x3 neighbours of every cell that references r; those
+// referencing it from afar are reached through the dependents box, the rest
+// lie within two cells of r.  So a neighbour tile is queued only if a
+// changed cell lies within two cells of the shared edge (or corner); the
+// tile itself is queued whenever it changed.
 __device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, int k, int tx, int ty,
-                                         const NextTiles &nt, int lane) {
+                                         const NextTiles &nt, unsigned long long imp, int lane) {
   if (lane < 9) {
-    const int X = tx + lane % 3 - 1, Y = ty + lane / 3 - 1;
-    return mark_issue(q, sc, k + 1, X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y);
+    const int dx = lane % 3 - 1, dy = lane / 3 - 1;
+    constexpr unsigned long long kCol01 = 0x0303030303030303ull, kRow01 = 0xFFFFull;
+    const unsigned long long selx = dx < 0 ? kCol01 : (dx > 0 ? kCol01 << (kTile - 2) : ~0ull);
+    const unsigned long long sely = dy < 0 ? kRow01 : (dy > 0 ? kRow01 << (kTile * (kTile - 2)) : ~0ull);
+    const int X = tx + dx, Y = ty + dy;
+    return mark_issue(q, sc, k + 1, (imp & selx & sely) != 0ull && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty,
+                      X, Y);
   }
   const int b = lane - 9;
   return mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
@@ -688,7 +706,10 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   uint64_t tp0 = a.prof ? wall_clock64() : 0;
 
   // The tile's dependents box (used only if the tile changes).
-  const uint32_t bx = lane < 4 ? ld_u32(reinterpret_cast<const uint32_t *>(&sc.box[tile]) + lane) : 0u;
+  // Lanes 0..3: the tile's dependents box; lanes 4, 5: its far-reference cells.
+  const uint32_t bx = lane < 4   ? ld_u32(reinterpret_cast<const uint32_t *>(&sc.box[tile]) + lane)
+                      : lane < 6 ? ld_u32(reinterpret_cast<const uint32_t *>(&sc.isref[tile]) + (lane - 4))
+                                 : 0u;
 
   // Stage the halo tile (load_points_into_shared_memory, propag.cu:231-261)
   // in two rounds of independent loads: the cells' state words and stamps,
@@ -753,6 +774,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   bool deferred = false;  // improved with a new reference whose walk is queued
   bool far_dep = false;  // improved with a new reference two or more cells away
   uint32_t rtile = 0;    // that reference's tile
+  uint32_t rbit = 0;     // and its cell's bit in the tile
   float best_t_out = 0.0f;
   uint32_t best_ref_out = 0u;
   if constexpr (kCache) {
@@ -807,12 +829,19 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   }
   // A changed tile issues the marks of its next-sweep tiles first, so their
   // round trip overlaps the line-of-sight walks below.
-  const bool changed = __ballot(improved) != 0ull;
+  const unsigned long long imp = __ballot(improved);
+  const bool changed = imp != 0ull;
+  const bool idle = __ballot(need) == 0ull;  // (a ballot of every lane)
+  if (lane == 0 && idle) ++cnt.idle;
   NextTiles nt{0, 0, 1, 0};
   Mark mk0{0u, 0u, 0u, false};
   if (changed) {
-    nt = next_tiles(sc, bx);
-    mk0 = next_round0_issue(a.q, sc, k, tx, ty, nt, lane);
+    // The far dependents need the next sweep only if a changed cell is
+    // some cell's far reference.
+    const unsigned long long refs =
+        static_cast<unsigned long long>(__shfl(bx, 4)) | (static_cast<unsigned long long>(__shfl(bx, 5)) << 32);
+    if (imp & refs) nt = next_tiles(sc, bx);
+    mk0 = next_round0_issue(a.q, sc, k, tx, ty, nt, imp, lane);
   }
   if (need) {
     const int me = lx + ly * kHalo;
@@ -837,6 +866,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       const int brx = ref_x<kCache>(best_ref), bry = ref_y<kCache>(best_ref);
       far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
       rtile = static_cast<uint32_t>(brx / kTile) + static_cast<uint32_t>(bry / kTile) * static_cast<uint32_t>(sc.ntx);
+      rbit = static_cast<uint32_t>((brx & 7) | ((bry & 7) << 3));
       if (!deferred) st_state(&st_out[gi], pack_state(best_t, best_ref));
       st_i32(&stamp[gi], k);
       ++cnt.improve;
@@ -866,14 +896,24 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     mark_commit(a.q, si, k + 1, mk0, lane);
     const uint32_t marks = next_rest(a.q, sc, si, k, nt, lane);
     if (lane == 0) cnt.marks += marks;
-    // Register this tile as a dependent of each new far reference's tile,
-    // one set of atomics per distinct reference tile.
+    // Register this tile as a dependent of each new far reference's tile
+    // and mark the reference cells, one set of atomics per distinct
+    // reference tile.
     unsigned long long m = __ballot(far_dep);
     while (m) {
       const int l = __builtin_ctzll(m);
       const uint32_t rt0 = __shfl(rtile, l);
-      m &= ~__ballot(far_dep && rtile == rt0);
+      const bool grp = far_dep && rtile == rt0;
+      m &= ~__ballot(grp);
+      uint32_t lo = (grp && rbit < 32u) ? 1u << rbit : 0u, hi = (grp && rbit >= 32u) ? 1u << (rbit - 32u) : 0u;
+      for (int off = 32; off > 0; off >>= 1) {
+        lo |= __shfl_xor(lo, off);
+        hi |= __shfl_xor(hi, off);
+      }
       if (lane == l) {
+        uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[rt0]);
+        if (lo) or_relaxed(r, lo);
+        if (hi) or_relaxed(r + 1, hi);
         uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt0]);
         min_relaxed(b + 0, static_cast<uint32_t>(tx));
         min_relaxed(b + 1, static_cast<uint32_t>(ty));
@@ -1139,13 +1179,13 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
 
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
-  unsigned long long v[12] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                              cnt.wait_ticks, 0ull, cnt.marks, 0ull, 0ull, 0ull, 0ull};
+  unsigned long long v[16] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
+                              cnt.wait_ticks, 0ull, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, 0ull, 0ull, 0ull};
   // phase totals are per wave (lane 0's copy only)
   const bool lead = (threadIdx.x & 63) == 0;
   for (int q = 0; q < 4; ++q) v[8 + q] = lead ? cnt.ph_total[q] : 0ull;
 #pragma unroll
-  for (int q = 0; q < 12; ++q) {
+  for (int q = 0; q < 16; ++q) {
     unsigned long long x = v[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0 && x) atomicAdd(&a.ctl->stats[q], x);
@@ -1181,7 +1221,7 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
   }
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
     const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, walkq, cnt);
@@ -1217,7 +1257,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], walkq[wave], cnt);
@@ -1249,10 +1289,12 @@ __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-__global__ void init_boxes(uint4 *box, size_t n) {
+__global__ void init_boxes(uint4 *box, unsigned long long *isref, size_t n) {
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     box[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+    isref[i] = 0ull;
+  }
 }
 
 __global__ void init_out(ScenOut *o, int k0) {
@@ -1305,6 +1347,9 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
         if (X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty) queue_tile(tq, sc, si, sc.k0, X, Y);
       }
       if (abs(static_cast<int>(rx[i]) - x) > 1 || abs(static_cast<int>(ry[i]) - y) > 1) {
+        const uint32_t rb = (rx[i] & 7u) | ((ry[i] & 7u) << 3);
+        or_relaxed(reinterpret_cast<uint32_t *>(&sc.isref[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]) + (rb >> 5),
+                   1u << (rb & 31u));
         uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]);
         min_relaxed(b + 0, static_cast<uint32_t>(tx));
         min_relaxed(b + 1, static_cast<uint32_t>(ty));
@@ -1458,6 +1503,7 @@ struct PerDevice {
   DeviceBuffer call, launch;
   std::vector<LaunchGraph *> graphs;
 };
+
 std::mutex g_dev_mu;
 std::vector<PerDevice *> g_dev;
 
@@ -1560,6 +1606,7 @@ size_t scen_buffer_bytes(size_t W, size_t H) {
   off = align_up(off + n * sizeof(int), 256);
   off = align_up(off + ntiles * sizeof(uint4), 256);
   off = align_up(off + ntiles * sizeof(unsigned long long), 256);
+  off = align_up(off + ntiles * sizeof(unsigned long long), 256);
   return off;
 }
 
@@ -1574,7 +1621,8 @@ ScenBuffers scen_carve(void *base, size_t W, size_t H) {
   b.stamp = reinterpret_cast<int *>(p + off); off = align_up(off + n * sizeof(int), 256);
   const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
   b.box = reinterpret_cast<uint4 *>(p + off); off = align_up(off + ntiles * sizeof(uint4), 256);
-  b.blk = reinterpret_cast<unsigned long long *>(p + off);
+  b.blk = reinterpret_cast<unsigned long long *>(p + off); off = align_up(off + ntiles * 8, 256);
+  b.isref = reinterpret_cast<unsigned long long *>(p + off);
   return b;
 }
 
@@ -1619,7 +1667,7 @@ pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenO
     return dim3(static_cast<unsigned>(b ? b : 1));
   };
   unsigned *err = &out->err;
-  hipLaunchKernelGGL(init_boxes, grid_for(ntiles), dim3(256), 0, stream, sc.box, ntiles);
+  hipLaunchKernelGGL(init_boxes, grid_for(ntiles), dim3(256), 0, stream, sc.box, sc.isref, ntiles);
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, err, static_cast<size_t>(1), 0u);
   hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, sc.g, speed, az, ecc,
                      boundaries);
@@ -1809,7 +1857,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       PG_HIP(hipGraphLaunch(lg->exec, stream));
       PG_HIP(hipStreamSynchronize(stream));
       std::memcpy(book, &lg->host->book, sizeof(book));
-      unsigned long long counters[12];
+      unsigned long long counters[16];
       std::memcpy(counters, lg->host->stats, sizeof(counters));
       float gms = 0.0f;
       PG_HIP(hipEventElapsedTime(&gms, lg->ev0, lg->ev1));
@@ -1822,7 +1870,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
   }
-  unsigned long long counters[12];
+  unsigned long long counters[16];
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
   float rms = 0.0f;
@@ -1848,6 +1896,7 @@ static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const
     stats->dda_steps += counters[2];
     stats->tiles_visited += counters[3];
     stats->tile_marks += counters[7];
+    stats->idle_tiles += counters[12];
     for (int q = 0; q < 4; ++q) stats->phase_total_ms[q] += static_cast<double>(counters[8 + q]) / clock_khz;
     stats->relax_ms += rms;
     stats->work_ms += static_cast<double>(counters[4]) / clock_khz / groups_d;
@@ -1907,6 +1956,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   sc.stamp = b.stamp;
   sc.box = b.box;
   sc.blk = b.blk;
+  sc.isref = b.isref;
   sc.bits = boundaries;
   sc.k0 = 0;
   TileQueue q;

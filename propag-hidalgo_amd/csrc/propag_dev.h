@@ -37,6 +37,7 @@ struct Scen {
   const uint32_t *bits;  // boundary bitset (row-major, the output layout)
   const unsigned long long *blk;  // the same bits in 8x8 blocks (walks)
   uint4 *box;            // per tile: tile box (x0, y0, x1, y1) of far dependents
+  unsigned long long *isref;  // per tile: its cells that are some cell's far reference (8x8 bits)
   uint32_t *mark[2];     // tiles queued for a sweep of parity 0 / 1
   int ntx, nty;          // tiles (= 8x8 bit blocks) per row / column
   uint32_t nwords_log2, pad0;
@@ -80,7 +81,8 @@ struct ScenBuffers {
   uint2 *st0, *st1;
   int *stamp;
   uint4 *box;
-  unsigned long long *blk;  // blocked boundary bitset
+  unsigned long long *blk;    // blocked boundary bitset
+  unsigned long long *isref;  // far-reference cells per tile
 };
 size_t scen_buffer_bytes(size_t width, size_t height);
 ScenBuffers scen_carve(void *base, size_t width, size_t height);
