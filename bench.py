@@ -46,6 +46,23 @@ FIELDS = ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed", "wind_azimuth
           "aspect")
 
 
+TRAFFIC_PROFILE = "profiles/propagate_batched_ensemble_1024_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the batched kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters cannot be read from
+    inside this process); None if it is absent."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -514,7 +531,8 @@ def main():
                   "config1_p99_ms": float(np.percentile(c1_lat, 99))},
         "autotune": tune_1024,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": TRAFFIC_PROFILE,
                      "kernel": "propagate_sweep" if args.mode == 1 else "propagate_persistent",
                      "kernel_ms": ens_relax_ms, "algorithmic_bytes": alg_bytes},
         "rothermel": {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,

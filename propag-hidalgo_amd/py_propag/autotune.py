@@ -35,8 +35,9 @@ class Autotuner:
     """Times candidate tunings of ``Context.run`` on sample ensembles and
     remembers the best per workload key."""
 
-    def __init__(self, ctx, space=None, repeats: int = 1):
+    def __init__(self, ctx, space=None, repeats: int = 1, clock=time.perf_counter):
         self.ctx = ctx
+        self.clock = clock
         self.space = space if space is not None else default_space()
         self.repeats = repeats
         self.best = {}     # key -> tuning dict
@@ -57,9 +58,9 @@ class Autotuner:
                 if i is not None:
                     self.ctx.release(i)
             raise RuntimeError("autotune sample does not fit the memory budget")
-        t0 = time.perf_counter()
+        t0 = self.clock()
         fin, stats = self.ctx.run(tuning=tuning)
-        dt = time.perf_counter() - t0
+        dt = self.clock() - t0
         sweeps = [self.ctx.status(i)[1] for i in ids]
         for i in ids:
             self.ctx.release(i)

@@ -47,7 +47,7 @@ class Server:
     ``Context.time_into``); the scenario is released afterwards."""
 
     def __init__(self, ctx: Context, on_done=None, target_ms: float = 2.0, min_sweeps: int = 16,
-                 max_sweeps: int = 4096, tuning=None, clock=time.perf_counter):
+                 max_sweeps: int = 4096, tuning=None, clock=time.perf_counter, sleep=time.sleep):
         self.ctx = ctx
         self.on_done = on_done
         self.target_ms = target_ms
@@ -55,6 +55,7 @@ class Server:
         self.sweeps = min_sweeps
         self.tuning = tuning
         self.clock = clock
+        self.sleep = sleep
         self.queue = collections.deque()
         self.running = {}            # scenario id -> Request
         self.launches = 0
@@ -127,7 +128,7 @@ class Server:
             if self.idle():
                 wait = pending[0].arrival - (self.clock() - start)
                 if wait > 0:
-                    time.sleep(min(wait, 0.001))
+                    self.sleep(min(wait, 0.001))
                 continue
             out.extend(self.step())
         return out
