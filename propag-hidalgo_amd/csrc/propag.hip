@@ -435,6 +435,7 @@ constexpr int kProfCols = 8;
 struct Counters {
   uint32_t relax, improve, steps, tiles, marks, idle;
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
+  uint64_t crit_ticks;              // the barrier master: slowest workgroup's work per sweep, summed
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
   uint64_t ph_total[4];             // profiling: this wave's phase ticks, all sweeps
 };
@@ -1143,7 +1144,7 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
           st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
           st_relaxed(&c->book.v[1], changed ? 0u : 1u);
         }
-        c->stats[6] += maxwork;  // only this lane ever writes it
+        cnt.crit_ticks += maxwork;  // written out once, at the end of the launch
         // Sweep k + 2 appends to list (k + 3) & 3, last read in sweep k - 1;
         // this store completes before this wave's next barrier arrival.
         st_relaxed(count_of(a.q, k + 3), 0u);
@@ -1180,7 +1181,7 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
   unsigned long long v[16] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                              cnt.wait_ticks, 0ull, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, 0ull, 0ull, 0ull};
+                              cnt.wait_ticks, cnt.crit_ticks, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, 0ull, 0ull, 0ull};
   // phase totals are per wave (lane 0's copy only)
   const bool lead = (threadIdx.x & 63) == 0;
   for (int q = 0; q < 4; ++q) v[8 + q] = lead ? cnt.ph_total[q] : 0ull;
@@ -1192,13 +1193,19 @@ __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   }
 }
 
+// Workgroup size bound of the persistent kernel: with the bitset in LDS at
+// most 8 waves fit beside it, so that form is bounded at 512 threads, which
+// lets it use up to 256 VGPRs (two waves per SIMD) instead of 128.
+template <bool kLdsBits>
+constexpr int persistent_threads() { return kLdsBits ? 512 : 1024; }
+
 // Persistent form: one launch runs sweeps k_begin.. until a sweep commits
 // nothing or k_end; workgroups meet at the grid barrier after each sweep.
 // List entry e belongs to workgroup e mod gridDim.x, so tiles queued
 // together (neighbours on the fire front) land on different CUs, and each
 // workgroup's waves draw its entries dynamically (sweep_wave_dyn).
 template <bool kCache, bool kLdsBits>
-__global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
+__global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_persistent(Args a) {
   // Per-sweep workgroup flags, double-buffered by sweep parity: sweep k
   // writes [k & 1], thread 0 clears it after barrier k, and it is written
   // again only in sweep k + 2, after barrier k + 1.
@@ -1221,7 +1228,7 @@ __global__ __launch_bounds__(1024) void propagate_persistent(Args a) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
   }
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
     const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, walkq, cnt);
@@ -1257,7 +1264,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], walkq[wave], cnt);
@@ -1824,7 +1831,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     // per-wave LDS: the tile slice, plus a walk queue when walks are deferred
     // (line-of-sight flags cached and the bitset read from L2)
     const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(WalkReq) * kWalkQ : 0);
-    const int max_waves = 16;  // the kernel's launch bound: 1024 threads
+    const int max_waves = (a.lds_bits ? persistent_threads<true>() : persistent_threads<false>()) / kWave;
     if (tn.waves_per_group) {
       // with the bitset in LDS, at most the slices that fit beside it
       waves = a.lds_bits ? std::min(waves, static_cast<int>(tn.waves_per_group)) : static_cast<int>(tn.waves_per_group);
