@@ -474,6 +474,24 @@ def main():
     burn_gbs = 49.0 * n / (burn_ms * 1e-3) / 1e9
     log(f"rothermel: {burn_ms * 1e3:.1f} us per 1024^2 pass, {burn_mcells:.0f} Mcells/s, {burn_gbs:.0f} GB/s")
 
+    # ------------------------------------------------ moisture pre-pass
+    mo_outs = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
+    mo_slope, mo_aspect = bc[FIELDS.index("slope")], bc[FIELDS.index("aspect")]
+    hours = np.arange(24)
+    mo_w = pp.day_weather(12 + 10 * np.sin(hours / 24 * 2 * np.pi), 60 - 25 * np.sin(hours / 24 * 2 * np.pi),
+                          30.0, [0.0, 0.0, 3.0, 0.0, 12.0, 0.0], 7)
+    for _ in range(3):
+        pp.moisture_device(bf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
+    e0.record(stream)
+    for _ in range(reps):
+        pp.moisture_device(bf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    mo_ms = e0.elapsed_time(e1) / reps
+    mo_gbs = 29.0 * n / (mo_ms * 1e-3) / 1e9
+    log(f"moisture pre-pass: {mo_ms * 1e3:.1f} us per 1024^2 hour, {n / (mo_ms * 1e-3) / 1e6:.0f} Mcells/s, "
+        f"{mo_gbs:.0f} GB/s")
+
     if args.copy_probe_mib:
         src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -541,6 +559,9 @@ def main():
         "serving": serving,
         "ensemble_2048": ens2048,
         "large_4096": large,
+        "moisture": {"value": n / (mo_ms * 1e-3) / 1e6, "unit": "Mcells/s", "us_per_pass": mo_ms * 1e3,
+                     "achieved_gbs": mo_gbs, "frac": mo_gbs / HBM_PEAK_GBS,
+                     "workload": "moisture pre-pass, 1024x1024 i.i.d. fuel/slope/aspect, July day, 14:00"},
         "cpu_baseline": cpu,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
                                                        "tiles_visited", "tile_marks", "idle_tiles", "relax_ms", "elapsed_ms",

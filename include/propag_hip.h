@@ -130,6 +130,29 @@ pg_status pg_burn(size_t n, const uint8_t *fuel, const float *d1hr,
                   const float *aspect, float *speed_max, float *azimuth_max,
                   float *eccentricity, void *stream);
 
+/* ------------------------------------------------- moisture pre-pass */
+
+/* One day of weather of a scenario: hourly temperature (C), relative
+ * humidity (%) and cloud cover (%), the rain (mm) of the six previous days
+ * (index 0 = the day before) and the month (1..12). */
+typedef struct pg_day_weather {
+  float temperature[24];
+  float humidity[24];
+  float cloud_cover[24];
+  float rain_6_days[6];
+  int32_t month;
+} pg_day_weather;
+
+/* Dead and live fuel moisture (ratios) of n cells at one hour (0..23) of a
+ * day: writes d1hr, d10hr, d100hr, herb and wood for the burn pass from the
+ * per-cell fuel code, slope (rise/run) and aspect (rad).  Replaces
+ * crates/firelib/src/fuel_moisture.rs:26-63 (create_terrain_with_fuel_moisture)
+ * and the dmoist model it calls (crates/dmoist/src/lib.rs:55-145), per cell
+ * on the device.  All arrays are device pointers. */
+pg_status pg_moisture(size_t n, const uint8_t *fuel, const float *slope, const float *aspect,
+                      const pg_day_weather *weather, int hour, float *d1hr, float *d10hr,
+                      float *d100hr, float *herb, float *wood, void *stream);
+
 /* ---------------------------------------------------------------- Stage P */
 
 /* Boundaries, relaxation to convergence and post-pass over one raster.

@@ -174,3 +174,61 @@ def sincos(x: float):
     s, c = C.c_double(), C.c_double()
     lib().oracle_sincos(C.c_double(x), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+# ------------------------------------------------------------ moisture
+
+
+class DayWeather(C.Structure):
+    _fields_ = [("temperature", C.c_float * 24), ("humidity", C.c_float * 24), ("cloud", C.c_float * 24),
+                ("rain", C.c_float * 6), ("month", C.c_int)]
+
+
+def day_weather(temperature, humidity, cloud, rain, month) -> DayWeather:
+    w = DayWeather()
+    for name, v, k in (("temperature", temperature, 24), ("humidity", humidity, 24), ("cloud", cloud, 24),
+                       ("rain", rain, 6)):
+        a = np.broadcast_to(np.asarray(v, np.float32), (k,))
+        getattr(w, name)[:] = [float(x) for x in a]
+    w.month = int(month)
+    return w
+
+
+def _moisture_fns():
+    lib_ = lib()
+    f = C.c_float
+    sig = {"oracle_rain_effect_max": (f, [C.c_int, C.POINTER(C.c_float)]),
+           "oracle_base_moisture": (C.c_int, [f, f]),
+           "oracle_shading": (f, [f, C.c_int]),
+           "oracle_ignition_probability": (f, [f, f, f, C.c_int]),
+           "oracle_corrected_moisture": (f, [f, f, C.c_int, f, f, C.c_int, C.c_int]),
+           "oracle_rain_corrected_probability": (f, [f, f]),
+           "oracle_fine_moisture": (f, [f, f, f]),
+           "oracle_thicker_moisture": (f, [C.c_int, f, f, f]),
+           "oracle_live_moisture": (f, [C.c_int])}
+    for name, (res, args) in sig.items():
+        fn = getattr(lib_, name)
+        fn.restype, fn.argtypes = res, args
+    return lib_
+
+
+def moisture_fn(name):
+    """One of the per-value moisture functions (oracle_<name>)."""
+    return getattr(_moisture_fns(), "oracle_" + name)
+
+
+def rain_effect_max(month, rain):
+    a = (C.c_float * 6)(*[float(x) for x in rain])
+    return float(_moisture_fns().oracle_rain_effect_max(int(month), a))
+
+
+def moisture(fuel, slope, aspect, weather: DayWeather, hour: int):
+    """Moisture ratios (d1hr, d10hr, d100hr, herb, wood) per cell at one hour."""
+    fuel = np.ascontiguousarray(fuel, np.uint8)
+    slope = np.ascontiguousarray(slope, np.float32)
+    aspect = np.ascontiguousarray(aspect, np.float32)
+    n = fuel.size
+    outs = [np.empty(n, np.float32) for _ in range(5)]
+    lib().oracle_moisture(C.c_size_t(n), _p(fuel), _p(slope), _p(aspect), C.byref(weather), C.c_int(hour),
+                          *[_p(o) for o in outs])
+    return tuple(outs)

@@ -110,6 +110,33 @@ int oracle_dda(int fx, int fy, int tx, int ty, int32_t *out_xy, int max_out);
  * machine compiled without contraction. */
 void oracle_sincos(double x, double *s, double *c);
 
+/* ---------------- moisture pre-pass (crates/dmoist/src/lib.rs) ------- */
+
+/* One day of weather of a scenario: hourly temperature (C), relative
+ * humidity (%), cloud cover (%), the rain (mm) of the six previous days
+ * (index 0 = the day before) and the month (1..12). */
+typedef struct oracle_day_weather {
+  float temperature[24], humidity[24], cloud[24];
+  float rain[6];
+  int month;
+} oracle_day_weather;
+
+float oracle_rain_effect_max(int month, const float rain[6]);
+int oracle_base_moisture(float temperature, float humidity);
+float oracle_shading(float cloud, int fuel);
+float oracle_ignition_probability(float temperature, float cloud, float moisture, int fuel);
+float oracle_corrected_moisture(float base, float cloud, int month, float aspect_deg, float slope_deg, int fuel,
+                                int hour);
+float oracle_rain_corrected_probability(float probability, float effect);
+float oracle_fine_moisture(float moisture, float prob_dry, float prob_wet);
+float oracle_thicker_moisture(int hour, float m, float m6, float m15);
+float oracle_live_moisture(int month);
+/* Moisture ratios (d1hr, d10hr, d100hr, herb, wood) of n cells at one hour,
+ * from per-cell fuel code, slope (rise/run) and aspect (rad). */
+void oracle_moisture(size_t n, const uint8_t *fuel, const float *slope, const float *aspect,
+                     const oracle_day_weather *w, int hour, float *d1hr, float *d10hr, float *d100hr, float *herb,
+                     float *wood);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,20 @@ pg_status pg_burn(size_t n, const uint8_t *fuel, const float *d1hr, const float 
                      aspect, speed_max, azimuth_max, eccentricity, static_cast<hipStream_t>(stream));
 }
 
+pg_status pg_moisture(size_t n, const uint8_t *fuel, const float *slope, const float *aspect,
+                      const pg_day_weather *weather, int hour, float *d1hr, float *d10hr, float *d100hr,
+                      float *herb, float *wood, void *stream) {
+  if (n == 0) return PG_OK;
+  if (!fuel || !slope || !aspect || !weather || !d1hr || !d10hr || !d100hr || !herb || !wood)
+    return set_error(PG_ERR_INVALID, "pg_moisture: null pointer");
+  if (hour < 0 || hour > 23) return set_error(PG_ERR_INVALID, "pg_moisture: hour must be 0..23");
+  DeviceInfo *dev = nullptr;
+  pg_status s = device_info(&dev);
+  if (s != PG_OK) return s;
+  return launch_moisture(dev, n, fuel, slope, aspect, weather, hour, d1hr, d10hr, d100hr, herb, wood,
+                         static_cast<hipStream_t>(stream));
+}
+
 pg_status pg_propagate(const Settings *settings, const float *speed_max, const float *azimuth_max,
                        const float *eccentricity, float *time, uint16_t *refs_x, uint16_t *refs_y,
                        uint32_t *boundaries, const pg_tuning *tuning, pg_stats *stats,

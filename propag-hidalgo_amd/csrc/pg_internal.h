@@ -80,6 +80,11 @@ pg_status launch_burn_weather(const DeviceInfo *dev, size_t n, const uint8_t *fu
                               const float *wind_speed_field, const float *wind_azimuth_field, float *speed,
                               float *az, float *ecc, hipStream_t stream);
 
+// Moisture pre-pass launcher (moisture.hip).
+pg_status launch_moisture(const DeviceInfo *dev, size_t n, const uint8_t *fuel, const float *slope,
+                          const float *aspect, const pg_day_weather *w, int hour, float *d1hr, float *d10hr,
+                          float *d100hr, float *herb, float *wood, hipStream_t stream);
+
 // Stage P driver (propag.hip).
 pg_status run_propagate(const DeviceInfo *dev, const Settings *settings,
                         const float *speed, const float *az, const float *ecc,

@@ -376,6 +376,37 @@ def burn_device(fuel, d1hr, d10hr, d100hr, herb, wood, wind_speed, wind_azimuth,
     _raise_for(rc, "pg_burn")
 
 
+def day_weather(temperature, humidity, cloud_cover, rain_6_days, month) -> _native.DayWeather:
+    """pg_day_weather of one day: hourly temperature (C), relative humidity
+    (%), cloud cover (%) (24 values or a scalar each), the rain (mm) of the
+    six previous days (index 0 = the day before) and the month (1..12)."""
+    w = _native.DayWeather()
+    for name, v, k in (("temperature", temperature, 24), ("humidity", humidity, 24),
+                       ("cloud_cover", cloud_cover, 24), ("rain_6_days", rain_6_days, 6)):
+        a = np.broadcast_to(np.asarray(v, np.float32), (k,))
+        getattr(w, name)[:] = [float(x) for x in a]
+    if not 1 <= int(month) <= 12:
+        raise PropagValidationError("month must be 1..12")
+    w.month = int(month)
+    return w
+
+
+def moisture_device(fuel, slope, aspect, weather: _native.DayWeather, hour: int, d1hr, d10hr, d100hr,
+                    herb, wood, stream=None):
+    """pg_moisture on resident tensors: the moisture ratios of every cell at
+    ``hour`` (0..23) of the day ``weather`` (see day_weather), written into
+    the five f32 outputs (the burn pass's moisture columns).  Asynchronous on
+    ``stream``."""
+    n = fuel.numel()
+    for t in (slope, aspect, d1hr, d10hr, d100hr, herb, wood):
+        if t.numel() != n:
+            raise PropagValidationError("all moisture rasters must have the same number of cells")
+    rc = _native.load().pg_moisture(n, _dptr(fuel), _dptr(slope), _dptr(aspect), C.byref(weather),
+                                    int(hour), _dptr(d1hr), _dptr(d10hr), _dptr(d100hr), _dptr(herb),
+                                    _dptr(wood), _stream_ptr(stream))
+    _raise_for(rc, "pg_moisture")
+
+
 def propagate_device(settings: Settings, speed_max, azimuth_max, eccentricity, time, refs_x,
                      refs_y, boundaries, tuning=None, stream=None) -> dict:
     """pg_propagate on resident tensors: time (f32) and refs (16-bit, e.g.

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpropag_hip.so")
 EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propagate",
            "pg_propagate_host", "pg_ctx_create", "pg_ctx_destroy", "pg_terrain_add",
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
-           "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished")
+           "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture")
 
 
 class GT(C.Structure):
@@ -53,6 +53,11 @@ class Tuning(C.Structure):
                 ("max_iterations", C.c_uint32), ("mode", C.c_uint32),
                 ("spin_limit_log2", C.c_uint32), ("waves_per_group", C.c_uint32),
                 ("profile", C.c_uint32)]
+
+
+class DayWeather(C.Structure):
+    _fields_ = [("temperature", C.c_float * 24), ("humidity", C.c_float * 24),
+                ("cloud_cover", C.c_float * 24), ("rain_6_days", C.c_float * 6), ("month", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -101,6 +106,8 @@ def load(require_torch: bool = True):
     lib.pg_catalog_f32.argtypes = [C.c_int, C.c_void_p]
     lib.pg_burn.restype = C.c_int
     lib.pg_burn.argtypes = [sz] + [vp] * 14
+    lib.pg_moisture.restype = C.c_int
+    lib.pg_moisture.argtypes = [sz, vp, vp, vp, C.POINTER(DayWeather), C.c_int] + [vp] * 6
     lib.pg_propagate.restype = C.c_int
     lib.pg_propagate.argtypes = [C.POINTER(SettingsC)] + [vp] * 7 + [C.POINTER(Tuning),
                                                                      C.POINTER(Stats), vp]
