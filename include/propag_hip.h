@@ -4,10 +4,12 @@
  * propagation.  Plain pointers and sizes only; every entry point returns a
  * pg_status and leaves a message for pg_last_error() on failure.
  *
- * The layouts of GT / GeoTransform / GeoReference / Settings are identical
- * to the reference's #[repr(C)] types so a caller can pass the same bytes:
+ * The layouts of GT / GeoTransform / GeoReference (geometry.h), Settings
+ * and the feature records are identical to the reference's #[repr(C)] types
+ * so a caller can pass the same bytes:
  *   GT, GeoTransform, GeoReference   crates/geometry/src/lib.rs:20-29,128-133,180-187
  *   Settings                         crates/propag/src/propag.rs:38-43
+ *   FFITimeFeature, FFIFuelFeature   crates/propag/src/propag.rs:120-125,150-155
  *
  * Device pointers are HIP device allocations on the current device; streams
  * are hipStream_t passed as void* (NULL = the default stream).
@@ -18,27 +20,13 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "geometry.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 /* ------------------------------------------------------------------ types */
-
-typedef struct GT {
-  float x0, dx, rx, y0, ry, dy;
-} GT;
-
-typedef struct GeoTransform {
-  GT gt;
-  GT inv;
-} GeoTransform;
-
-typedef struct GeoReference {
-  uint32_t width;
-  uint32_t height;
-  uint8_t proj[1024];
-  GeoTransform transform;
-} GeoReference;
 
 typedef struct Settings {
   GeoReference geo_ref;
@@ -187,6 +175,47 @@ pg_status pg_propagate_host(const Settings *settings, const uint8_t *fuel,
                             uint16_t *refs_x, uint16_t *refs_y,
                             uint32_t *boundaries, float *fire_out,
                             const pg_tuning *tuning, pg_stats *stats);
+
+/* ------------------------------------- initial state (host, no GPU) */
+
+/* An ignited feature: a time (s) and its geometry as WKB (caller-owned). */
+typedef struct FFITimeFeature {
+  float time;
+  const uint8_t *geom_wkb;
+  uintptr_t geom_wkb_len;
+} FFITimeFeature;
+
+/* A fuel feature: an NFFL fuel code and its geometry as WKB. */
+typedef struct FFIFuelFeature {
+  uint8_t code;
+  const uint8_t *geom_wkb;
+  uintptr_t geom_wkb_len;
+} FFIFuelFeature;
+
+/* Burns n ignition features into time (W*H host array, row-major in the
+ * layout of geo): the raster is filled with FLT_MAX, then every feature
+ * burns its time, later features replacing earlier ones, by GDAL's default
+ * rules (points: the pixel containing them; lines: a Bresenham line between
+ * the pixels of consecutive vertices; polygons and collections: the pixels
+ * whose centres are inside, scan line by scan line).  Geometries are WKB
+ * (OGC or ISO, 2-D/Z/M/ZM, either byte order) in the raster's coordinates;
+ * a feature whose WKB does not parse is skipped, as the reference does.
+ * *burned (may be NULL) receives the number of features that parsed.
+ * Replaces rasterize_times (crates/propag/src/propag.rs:837-890), which
+ * calls GDALRasterizeGeometries.  Host code; needs no GPU. */
+pg_status pg_rasterize_times(const GeoReference *geo, const FFITimeFeature *features,
+                             size_t n, float *time, uint32_t *burned);
+
+/* The same for fuel features into a W*H u8 raster pre-filled with 255:
+ * replaces rasterize_fuels (crates/propag/src/propag.rs:929-978). */
+pg_status pg_rasterize_fuels(const GeoReference *geo, const FFIFuelFeature *features,
+                             size_t n, uint8_t *fuel, uint32_t *burned);
+
+/* Initial references of propagate() (crates/propag/src/propag.rs:433-449):
+ * each cell whose time is not FLT_MAX points at itself, every other cell
+ * holds 0xFFFF.  W*H host arrays; needs no GPU. */
+pg_status pg_init_refs(const GeoReference *geo, const float *time, uint16_t *refs_x,
+                       uint16_t *refs_y);
 
 /* ---------------------------------------------- batching and serving */
 

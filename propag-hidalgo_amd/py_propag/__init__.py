@@ -5,9 +5,9 @@ __init__.py:6-29 and crates/py-propag/src/lib.rs): ``GeoReference``,
 ``Settings``, ``TimeFeature``, ``PropagResults``, ``propagate``,
 ``load_terrain_data``, ``check_cuda_available`` and the exception types.
 Unlike the reference's mock (py-propag/src/lib.rs:676-691), ``propagate``
-runs the real burn + propagation on the GPU through libpropag_hip.so.  Output
-files (GeoTIFF/GPKG) and WKB rasterisation are outside this package; ignitions
-are given as raster cells.
+runs the real burn + propagation on the GPU through libpropag_hip.so.
+Ignitions are WKB features (rasterised by the library with GDAL's default
+burn rules, ``rasterize_times``), raster cells, or both.
 
 Two levels:
   * host arrays (numpy): ``burn``, ``propagate`` -- the whole device data
@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -178,10 +179,10 @@ class Settings:
 
 @dataclass
 class TimeFeature:
-    """An ignition: a time (s) and either WKB geometry or raster cells.
-
-    Only ``cells`` (an iterable of (x, y)) is rasterised here; WKB input
-    needs the GDAL rasteriser of the reference (propag.rs:837-890)."""
+    """An ignition: a time (s) and WKB geometry (TimeFeature, propag.rs:115-118)
+    and/or raster cells (an iterable of (x, y)).  WKB is in the raster's
+    coordinates and burns as the reference's rasterize_times does
+    (propag.rs:837-890)."""
     time: float
     geom_wkb: bytes = b""
     cells: list = field(default_factory=list)
@@ -269,9 +270,86 @@ def _terrain_columns(terrain: dict, n: int):
     return fuel, cols
 
 
-def initial_state(width: int, height: int, ignitions):
-    """time / refs as propag.rs:433-449 builds them from ignited cells.
-    ``ignitions``: iterable of (x, y, t) or TimeFeature with ``cells``."""
+def _feature_array(features, cls, value_of):
+    """ctypes array of FFI feature records; keeps the WKB bytes alive."""
+    keep = [bytes(wkb) for _, wkb in features]
+    arr = (cls * max(1, len(features)))()
+    for i, ((v, _), b) in enumerate(zip(features, keep)):
+        setattr(arr[i], cls._fields_[0][0], value_of(v))
+        arr[i].geom_wkb = C.cast(C.c_char_p(b), C.c_void_p) if b else None
+        arr[i].geom_wkb_len = len(b)
+    return arr, keep
+
+
+def rasterize_times(geo_ref: GeoReference, features):
+    """Burns (time, WKB) features into a (height, width) f32 raster (FLT_MAX
+    where nothing burned) with GDAL's default rules, later features
+    replacing earlier ones; returns (raster, number of features that
+    parsed).  Replaces rasterize_times (crates/propag/src/propag.rs:837-890)."""
+    feats = [(f.time, f.geom_wkb) if isinstance(f, TimeFeature) else tuple(f) for f in features]
+    arr, keep = _feature_array(feats, _native.TimeFeatureC, float)
+    out = np.empty((geo_ref.height, geo_ref.width), np.float32)
+    n = C.c_uint32(0)
+    lib = _native.load(require_torch=False)
+    g = geo_ref.to_c()
+    _raise_for(lib.pg_rasterize_times(C.byref(g), arr, len(feats), _ptr(out), C.byref(n)),
+               "pg_rasterize_times")
+    return out, int(n.value)
+
+
+def rasterize_fuels(geo_ref: GeoReference, features):
+    """Burns (fuel code, WKB) features into a (height, width) u8 raster (255
+    where nothing burned): propag_rasterize_fuels (propag.rs:893-978)."""
+    feats = [tuple(f) for f in features]
+    for code, _ in feats:
+        if not 0 <= int(code) <= 255:
+            raise PropagValidationError(f"fuel code {code} does not fit a byte")
+    arr, keep = _feature_array(feats, _native.FuelFeatureC, int)
+    out = np.empty((geo_ref.height, geo_ref.width), np.uint8)
+    n = C.c_uint32(0)
+    lib = _native.load(require_torch=False)
+    g = geo_ref.to_c()
+    _raise_for(lib.pg_rasterize_fuels(C.byref(g), arr, len(feats), _ptr(out), C.byref(n)),
+               "pg_rasterize_fuels")
+    return out, int(n.value)
+
+
+def _point_wkb(x: float, y: float) -> bytes:
+    return struct.pack("<BIdd", 1, 1, x, y)
+
+
+def initial_state(width: int, height: int, ignitions, geo_ref: GeoReference | None = None):
+    """time / refs as propag.rs:433-449 builds them from the ignitions.
+    ``ignitions``: iterable of (x, y, t) cells or TimeFeature.  When any
+    TimeFeature carries WKB (needs ``geo_ref``), every ignition burns through
+    the library's rasteriser in the given order (cells as points at their
+    centres) and the references come from pg_init_refs."""
+    ignitions = list(ignitions)
+    if any(isinstance(i, TimeFeature) and i.geom_wkb for i in ignitions):
+        if geo_ref is None or (geo_ref.width, geo_ref.height) != (width, height):
+            raise PropagValidationError("WKB ignitions need the raster's GeoReference")
+        gt = [float(v) for v in geo_ref.transform]
+        feats = []
+        for ign in ignitions:
+            if isinstance(ign, TimeFeature):
+                if ign.geom_wkb:
+                    feats.append((ign.time, ign.geom_wkb))
+                pts = [(x, y, ign.time) for x, y in ign.cells]
+            else:
+                pts = [ign]
+            for x, y, t in pts:
+                if not (0 <= x < width and 0 <= y < height):
+                    raise PropagValidationError(f"ignition cell ({x}, {y}) is outside the raster")
+                cx, cy = x + 0.5, y + 0.5
+                feats.append((t, _point_wkb(gt[0] + cx * gt[1] + cy * gt[2], gt[3] + cx * gt[4] + cy * gt[5])))
+        time, _ = rasterize_times(geo_ref, feats)
+        time = time.ravel()
+        rx = np.empty(width * height, np.uint16)
+        ry = np.empty(width * height, np.uint16)
+        lib = _native.load(require_torch=False)
+        g = geo_ref.to_c()
+        _raise_for(lib.pg_init_refs(C.byref(g), _ptr(time), _ptr(rx), _ptr(ry)), "pg_init_refs")
+        return time, rx, ry
     n = width * height
     time = np.full(n, FLT_MAX, np.float32)
     rx = np.full(n, NO_REF, np.uint16)
@@ -322,7 +400,7 @@ def propagate(settings: Settings, terrain: dict, ignitions=None, time=None, refs
     if time is None:
         if not ignitions:
             raise PropagValidationError("At least one ignition element is required")
-        time, refs_x, refs_y = initial_state(g.width, g.height, ignitions)
+        time, refs_x, refs_y = initial_state(g.width, g.height, ignitions, g)
     time = np.ascontiguousarray(np.asarray(time, np.float32).ravel()).copy()
     refs_x = np.ascontiguousarray(np.asarray(refs_x, np.uint16).ravel()).copy()
     refs_y = np.ascontiguousarray(np.asarray(refs_y, np.uint16).ravel()).copy()
@@ -449,7 +527,8 @@ __all__ = [
     "Settings", "GeoReference", "TimeFeature", "PropagResults", "propagate", "load_terrain_data",
     "check_cuda_available", "check_gpu_available", "PropagCudaError", "PropagGpuError",
     "PropagGdalError", "PropagValidationError", "PropagMPIError", "PropagNotConverged", "burn",
-    "burn_device", "propagate_device", "initial_state", "Tuning", "Stats", "FLT_MAX", "NO_REF",
+    "burn_device", "propagate_device", "initial_state", "rasterize_times", "rasterize_fuels",
+    "Tuning", "Stats", "FLT_MAX", "NO_REF",
 ]
 
 

@@ -16,12 +16,23 @@ LIB_PATH = os.path.join(PKG_DIR, "libpropag_hip.so")
 EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propagate",
            "pg_propagate_host", "pg_ctx_create", "pg_ctx_destroy", "pg_terrain_add",
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
-           "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture")
+           "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture",
+           "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs")
 
 
 class GT(C.Structure):
     _fields_ = [("x0", C.c_float), ("dx", C.c_float), ("rx", C.c_float),
                 ("y0", C.c_float), ("ry", C.c_float), ("dy", C.c_float)]
+
+
+class TimeFeatureC(C.Structure):
+    """FFITimeFeature (crates/propag/src/propag.rs:120-125)."""
+    _fields_ = [("time", C.c_float), ("geom_wkb", C.c_void_p), ("geom_wkb_len", C.c_size_t)]
+
+
+class FuelFeatureC(C.Structure):
+    """FFIFuelFeature (crates/propag/src/propag.rs:150-155)."""
+    _fields_ = [("code", C.c_uint8), ("geom_wkb", C.c_void_p), ("geom_wkb_len", C.c_size_t)]
 
 
 class GeoTransform(C.Structure):
@@ -130,6 +141,14 @@ def load(require_torch: bool = True):
         f.argtypes = args
     lib.pg_scenario_bytes.restype = sz
     lib.pg_scenario_bytes.argtypes = [u32, u32]
+    lib.pg_rasterize_times.restype = C.c_int
+    lib.pg_rasterize_times.argtypes = [C.POINTER(GeoReferenceC), C.POINTER(TimeFeatureC), sz, vp,
+                                       C.POINTER(u32)]
+    lib.pg_rasterize_fuels.restype = C.c_int
+    lib.pg_rasterize_fuels.argtypes = [C.POINTER(GeoReferenceC), C.POINTER(FuelFeatureC), sz, vp,
+                                       C.POINTER(u32)]
+    lib.pg_init_refs.restype = C.c_int
+    lib.pg_init_refs.argtypes = [C.POINTER(GeoReferenceC), vp, vp, vp]
     _lib = lib
     return lib
 
