@@ -41,7 +41,8 @@ typedef enum pg_status {
   PG_ERR_NOT_CONVERGED = 4, /* the iteration cap was reached                */
   PG_ERR_OUT_OF_MEMORY = 5, /* device allocation or admission refused       */
   PG_ERR_TIMEOUT = 6,     /* a device-side wait exceeded its spin limit     */
-  PG_ERR_BUSY = 7         /* serving queue full / slot unavailable          */
+  PG_ERR_BUSY = 7,        /* serving queue full / slot unavailable          */
+  PG_ERR_IO = 8           /* an output file could not be written            */
 } pg_status;
 
 /* Propagation schedule knobs.  Zero fields take the library defaults. */
@@ -216,6 +217,28 @@ pg_status pg_rasterize_fuels(const GeoReference *geo, const FFIFuelFeature *feat
  * holds 0xFFFF.  W*H host arrays; needs no GPU. */
 pg_status pg_init_refs(const GeoReference *geo, const float *time, uint16_t *refs_x,
                        uint16_t *refs_y);
+
+/* ------------------------------------------- output files (host) */
+
+/* Writes a W*H arrival-time raster (FLT_MAX = not burned) as a GeoTIFF:
+ * one f32 band, 256 x 256 tiles, uncompressed, nodata FLT_MAX, the
+ * raster's geotransform and CRS (BigTIFF when the file would pass 4 GiB).
+ * Replaces write_times (crates/propag/src/propag.rs:576-600). */
+pg_status pg_write_times_geotiff(const char *path, const GeoReference *geo, const float *time);
+
+/* Writes a GeoPackage with layer "fire_references": for every cell whose
+ * time is below FLT_MAX, visited column by column, a line from the centre
+ * of its reference cell to its own centre.  Replaces write_refs
+ * (crates/propag/src/propag.rs:699-750).  Uses the system libsqlite3. */
+pg_status pg_write_refs_gpkg(const char *path, const GeoReference *geo, const float *time,
+                             const uint16_t *refs_x, const uint16_t *refs_y);
+
+/* Writes a GeoPackage with a polygon layer named `layer` of the rectangles
+ * of tile_w x tile_h cells that cover the raster, row by row (the launch
+ * decomposition: "blocks" are the kernel's tiles, "grids" its launch
+ * domains).  Replaces write_boundaries (crates/propag/src/propag.rs:601-698). */
+pg_status pg_write_tiles_gpkg(const char *path, const char *layer, const GeoReference *geo,
+                              uint32_t tile_w, uint32_t tile_h);
 
 /* ---------------------------------------------- batching and serving */
 

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import struct
 from dataclasses import dataclass, field
 
@@ -66,6 +67,8 @@ def _raise_for(status: int, what: str):
         raise PropagValidationError(msg)
     if status == 4:
         raise PropagNotConverged(msg)
+    if status == 8:
+        raise PropagGdalError(msg)     # the reference's error type for its output files
     raise PropagCudaError(msg)
 
 
@@ -314,6 +317,36 @@ def rasterize_fuels(geo_ref: GeoReference, features):
     return out, int(n.value)
 
 
+def write_times_geotiff(path, geo_ref: GeoReference, time) -> None:
+    """Arrival times as the reference's GeoTIFF (pg_write_times_geotiff)."""
+    t = np.ascontiguousarray(np.asarray(time, np.float32).reshape(-1))
+    if t.size != len(geo_ref):
+        raise PropagValidationError("time raster does not match the geo reference")
+    g = geo_ref.to_c()
+    _raise_for(_native.load(require_torch=False).pg_write_times_geotiff(os.fsencode(path), C.byref(g), _ptr(t)),
+               "pg_write_times_geotiff")
+
+
+def write_refs_gpkg(path, geo_ref: GeoReference, time, refs_x, refs_y) -> None:
+    """The reference's "fire_references" GeoPackage layer (pg_write_refs_gpkg)."""
+    arrs = [np.ascontiguousarray(np.asarray(a, dt).reshape(-1))
+            for a, dt in ((time, np.float32), (refs_x, np.uint16), (refs_y, np.uint16))]
+    if any(a.size != len(geo_ref) for a in arrs):
+        raise PropagValidationError("rasters do not match the geo reference")
+    g = geo_ref.to_c()
+    _raise_for(_native.load(require_torch=False).pg_write_refs_gpkg(os.fsencode(path), C.byref(g),
+                                                                    *[_ptr(a) for a in arrs]),
+               "pg_write_refs_gpkg")
+
+
+def write_tiles_gpkg(path, layer: str, geo_ref: GeoReference, tile_w: int, tile_h: int) -> None:
+    """A "blocks" / "grids" polygon layer (pg_write_tiles_gpkg)."""
+    g = geo_ref.to_c()
+    _raise_for(_native.load(require_torch=False).pg_write_tiles_gpkg(os.fsencode(path), layer.encode(), C.byref(g),
+                                                                     int(tile_w), int(tile_h)),
+               "pg_write_tiles_gpkg")
+
+
 def _point_wkb(x: float, y: float) -> bytes:
     return struct.pack("<BIdd", 1, 1, x, y)
 
@@ -528,7 +561,7 @@ __all__ = [
     "check_cuda_available", "check_gpu_available", "PropagCudaError", "PropagGpuError",
     "PropagGdalError", "PropagValidationError", "PropagMPIError", "PropagNotConverged", "burn",
     "burn_device", "propagate_device", "initial_state", "rasterize_times", "rasterize_fuels",
-    "Tuning", "Stats", "FLT_MAX", "NO_REF",
+    "write_times_geotiff", "write_refs_gpkg", "write_tiles_gpkg", "Tuning", "Stats", "FLT_MAX", "NO_REF",
 ]
 
 

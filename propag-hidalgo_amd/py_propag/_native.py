@@ -17,7 +17,8 @@ EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propa
            "pg_propagate_host", "pg_ctx_create", "pg_ctx_destroy", "pg_terrain_add",
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
            "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture",
-           "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs")
+           "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs", "pg_write_times_geotiff",
+           "pg_write_refs_gpkg", "pg_write_tiles_gpkg")
 
 
 class GT(C.Structure):
@@ -91,7 +92,7 @@ class Stats(C.Structure):
 
 STATUS = {0: "PG_OK", 1: "PG_ERR_INVALID", 2: "PG_ERR_HIP", 3: "PG_ERR_NO_DEVICE",
           4: "PG_ERR_NOT_CONVERGED", 5: "PG_ERR_OUT_OF_MEMORY", 6: "PG_ERR_TIMEOUT",
-          7: "PG_ERR_BUSY"}
+          7: "PG_ERR_BUSY", 8: "PG_ERR_IO"}
 
 _lib = None
 
@@ -149,6 +150,12 @@ def load(require_torch: bool = True):
                                        C.POINTER(u32)]
     lib.pg_init_refs.restype = C.c_int
     lib.pg_init_refs.argtypes = [C.POINTER(GeoReferenceC), vp, vp, vp]
+    lib.pg_write_times_geotiff.restype = C.c_int
+    lib.pg_write_times_geotiff.argtypes = [C.c_char_p, C.POINTER(GeoReferenceC), vp]
+    lib.pg_write_refs_gpkg.restype = C.c_int
+    lib.pg_write_refs_gpkg.argtypes = [C.c_char_p, C.POINTER(GeoReferenceC), vp, vp, vp]
+    lib.pg_write_tiles_gpkg.restype = C.c_int
+    lib.pg_write_tiles_gpkg.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(GeoReferenceC), u32, u32]
     _lib = lib
     return lib
 
