@@ -217,6 +217,19 @@ __device__ inline float time_from(float from_t, int fx, int fy, int tx, int ty, 
   return from_t + (dist * (1.0f - proj / r)) * k.z;
 }
 
+// time_from from the neighbour (tx + i, ty + j), |i|, |j| <= 1, to (tx, ty):
+// the same operations as time_from, whose radius sqrtf(ux^2 + uy^2) is then
+// sqrtf(1) = 1 or sqrtf(2), both exact constants, on square cells.
+__device__ inline float time_from_adjacent(float from_t, int i, int j, int tx, int ty, const float4 &k,
+                                           const Geom &g) {
+  if (!g.square) return time_from(from_t, tx + i, ty + j, tx, ty, k, g);
+  const float ux = static_cast<float>(-i);
+  const float uy = static_cast<float>(g.north_up ? j : -j);
+  const float r = (i != 0 && j != 0) ? 0x1.6a09e6p+0f : 1.0f;  // sqrtf(2.0f) correctly rounded
+  const float proj = k.x * uy + k.y * ux;
+  return from_t + ((r - proj) * g.cell) * k.z;
+}
+
 // Reference words.  With both raster sides below 32768 (the usual case) a
 // reference is x | y << 16 with 15-bit coordinates and bit 15 caching
 // whether the DDA line from the cell to its reference is free of boundary
@@ -234,8 +247,8 @@ __device__ inline uint32_t make_ref(int x, int y) {
   return static_cast<uint32_t>(x) | (static_cast<uint32_t>(y) << 16);
 }
 
-// Dynamic LDS of the persistent kernel: per-wave tile slices, then the
-// boundary bitset when it fits.
+// Dynamic LDS of the persistent kernel: the boundary bitset when it fits
+// (at offset 0), then the per-wave tile slices.
 extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
 
 // Where a walk reads the boundary bits.  From global memory (L2) it reads a
@@ -248,7 +261,7 @@ extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
 // selecting among blocks.
 struct BitView {
   const unsigned long long *blk;  // blocked copy (global form)
-  unsigned lds_off;               // row-major bitset in dyn_lds (LDS form)
+  unsigned lds_off;               // unused: the LDS form's row-major bitset starts dyn_lds
   int W, bw, bh;
 };
 
@@ -261,7 +274,7 @@ template <bool kLdsBits>
 __device__ inline bool bit_at(const BitView &bv, int x, int y) {
   if (kLdsBits) {
     const uint32_t i = static_cast<uint32_t>(x) + static_cast<uint32_t>(y) * static_cast<uint32_t>(bv.W);
-    return (dyn_lds[bv.lds_off + (i >> 5)] >> (i & 31u)) & 1u;
+    return (dyn_lds[i >> 5] >> (i & 31u)) & 1u;
   }
   const unsigned long long w = G(bv.blk)[static_cast<uint32_t>((x >> 3) + (y >> 3) * bv.bw)];
   return (w >> ((x & 7) | ((y & 7) << 3))) & 1ull;
@@ -293,10 +306,44 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const BitView &bv, 
   const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ru)));
   const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(rv)));
   float tmx = dlx, tmy = dly;
+  // An axis without displacement never steps (its tmax is infinite), so it
+  // cannot end the walk; pu / pv lift it out of the all-live test below.
+  const int pu = ddx == 0 ? (1 << 24) : 0, pv = ddy == 0 ? (1 << 24) : 0;
   if constexpr (kLdsBits) {
     const int iy = sy * bv.W;
     int idx = fx + fy * bv.W;
     for (;;) {
+      // ru and rv drop by at most one per cell, so with both at 7 or more all
+      // eight cells of the batch lie on the walk: no per-cell end test, and
+      // the bits are only OR-ed (bit 0 of each shifted word) until one is
+      // set.  Taken only when every walking lane of the wave qualifies, so
+      // the wave never runs both forms of the batch.
+      if (__ballot(min(ru + pu, rv + pv) >= 7) == __activemask()) {
+        uint32_t id[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          id[b] = static_cast<uint32_t>(idx);
+          const float d = tmx - tmy;
+          const bool xs = d <= kTie, ys = !(d < -kTie);
+          ru -= xs ? 1 : 0;
+          rv -= ys ? 1 : 0;
+          idx += (xs ? sx : 0) + (ys ? iy : 0);
+          tmx = xs ? tmx + dlx : tmx;
+          tmy = ys ? tmy + dly : tmy;
+        }
+        uint32_t acc = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc |= dyn_lds[id[b] >> 5] >> (id[b] & 31u);
+        if (acc & 1u) {
+          uint32_t hits = 0;
+#pragma unroll
+          for (int b = 0; b < 8; ++b) hits |= ((dyn_lds[id[b] >> 5] >> (id[b] & 31u)) & 1u) << b;
+          steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
+          return true;
+        }
+        steps += 8;
+        continue;
+      }
       uint32_t id[8];
       int live[8];
 #pragma unroll
@@ -314,7 +361,7 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const BitView &bv, 
       uint32_t hits = 0;
 #pragma unroll
       for (int b = 0; b < 8; ++b)
-        hits |= (((dyn_lds[bv.lds_off + (id[b] >> 5)] >> (id[b] & 31u)) & 1u) && live[b] >= 0) ? (1u << b) : 0u;
+        hits |= (((dyn_lds[id[b] >> 5] >> (id[b] & 31u)) & 1u) && live[b] >= 0) ? (1u << b) : 0u;
       if (hits) {
         steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
         return true;
@@ -656,7 +703,7 @@ __device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, in
       const int rfx = ref_x<true>(ref), rfy = ref_y<true>(ref);
       if (rfx != tx || rfy != ty) {  // a neighbour referencing this cell offers nothing
         const float4 kf = make_float4(s.ca[nl], s.cb[nl], s.cr[nl], 0.0f);
-        const float nt = time_from(s.time[nl], tx + i, ty + j, tx, ty, kf, g);
+        const float nt = time_from_adjacent(s.time[nl], i, j, tx, ty, kf, g);
         const float ct = time_from(s.rtime[nl], rfx, rfy, tx, ty, kf, g);
         const bool use_ref = ct < nt && (ref & kClearBit) != 0u;
         const float cand = use_ref ? ct : nt;
@@ -1214,18 +1261,20 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   __shared__ unsigned long long s_ph[2][4];
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
-  // Dynamic LDS: nw tile slices, then (deferred walks) nw walk queues, then
-  // (LDS bitset) the boundary bitset.
-  TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds);
+  // Dynamic LDS: (LDS bitset) the boundary bitset at offset 0, so a walk's
+  // word address needs no base; then nw tile slices and (deferred walks) nw
+  // walk queues.
+  const unsigned bits_words = kLdsBits ? (a.lds_words + 3u) & ~3u : 0u;  // 16-byte aligned slices
+  TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds + bits_words);
   TileLds &s = slices[wave];
   constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
   WalkReq *walkq = reinterpret_cast<WalkReq *>(slices + nw) + wave * kWalkQ;
-  const unsigned lds_off = nw * static_cast<unsigned>((sizeof(TileLds) + (kDefer ? sizeof(WalkReq) * kWalkQ : 0)) / 4);
+  const unsigned lds_off = 0;
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
   if (threadIdx.x < 2) s_next[threadIdx.x] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   if (kLdsBits) {
-    for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[lds_off + i] = a.lds_src[i];
+    for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
   }
   __syncthreads();
   Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -1818,7 +1867,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   } else {
     // Waves per workgroup: as many tile slices as fit beside the bitset when
     // the bitset can live in LDS (one workgroup per CU), else 16 waves.
-    const size_t bits_bytes = L.lds_bits_words * sizeof(uint32_t);
+    const size_t bits_bytes = ((L.lds_bits_words + 3) & ~size_t{3}) * sizeof(uint32_t);  // padded as the kernel lays it out
     const size_t static_lds = 256;
     int waves = 16;
     a.lds_bits = 0;
