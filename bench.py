@@ -446,7 +446,8 @@ def main():
         f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle), {es['dda_steps']} walk steps; "
         f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
         f"phase totals over waves {es['phase_total_ms']}, "
-        f"barrier (start/span/gather/book) {es['barrier_ms']}")
+        f"barrier (start/span/gather/book) {es['barrier_ms']}; per workgroup: work {es['work_ms']:.1f} ms, "
+        f"wait {es['wait_ms']:.1f} ms, slowest-workgroup work summed over sweeps {es['crit_ms']:.1f} ms")
     serving = serve_stream(args, pp, synth, torch, tuning) if args.serve_seconds > 0 else None
     ens2048 = ensemble_2048_section(args, pp, synth, torch, tuning, stream) if args.batch2048 > 0 else None
     large = None if args.no_large else large_single(args, pp, synth, torch, tuning, stream)
@@ -565,7 +566,7 @@ def main():
         "cpu_baseline": cpu,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
                                                        "tiles_visited", "tile_marks", "idle_tiles", "relax_ms", "elapsed_ms",
-                                                       "workgroups")},
+                                                       "workgroups", "work_ms", "wait_ms", "crit_ms")},
                         "config1": {"sweeps": st1["iterations"], "relaxations": st1["relaxations"],
                                     "improvements": st1["improvements"], "dda_steps": st1["dda_steps"],
                                     "tiles_visited": st1["tiles_visited"], "burned_cells": burned,

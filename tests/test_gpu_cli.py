@@ -1,0 +1,42 @@
+"""The `propag` command-line program (the reference's crates/propag/src/main.rs
+on the MI355X path): its fixed README scenario - 1024 x 1024, 5 m, fuel
+model 1, moisture 0.1, wind 5 m/s toward 0, point ignition at (500, 500),
+10 h.  The reference printed num_times_after=447822 and max_time 35999.9
+(README.md:92-113) from its older tiled build; the exact-schedule result is
+450,599 burned cells, the closed-form ellipse count that the oracle is pinned
+to (tests/test_oracle_propag.py) and pg_propagate reproduces
+(tests/test_gpu_propagate.py).  The GeoTIFF it writes must hold the same
+burned cells and extreme times as it prints.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from tiffread import read_tiff
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "propag-hidalgo_amd", "propag")
+F = np.float32(np.finfo(np.float32).max)
+
+
+def test_cli_runs_the_readme_scenario(tmp_path):
+    out = tmp_path / "tiempos.tif"
+    r = subprocess.run([CLI, str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert "fire_pos=(500, 500)" in lines and "config_max_time=36000" in lines
+    maxes = [float(m.group(1)) for m in re.finditer(r"^max_time=Some\(([-0-9.]+)\)$", r.stdout, re.M)]
+    burned = int(re.search(r"^num_times_after=(\d+)$", r.stdout, re.M).group(1))
+    assert burned == 450599
+    assert abs(maxes[0] - 35999.9) < 1.0 and maxes[1] == 0.0
+    tags, t = read_tiff(out)
+    assert t.shape == (1024, 1024)
+    lit = t < F
+    assert int(lit.sum()) == burned
+    assert abs(float(t[lit].max()) - maxes[0]) < 0.051 and float(t[lit].min()) == 0.0
+    assert t[500, 500] == 0.0
