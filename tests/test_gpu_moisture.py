@@ -76,3 +76,46 @@ def test_moisture_rejects_bad_arguments():
     f = torch.zeros(16, dtype=torch.uint8, device="cuda:0")
     with pytest.raises(pp.PropagValidationError):
         pp.moisture_device(f, t, t, w, 24, t, t, t, t, t)
+
+
+def test_moisture_feeds_the_burn_pass(oracle_lib):
+    """The moisture rasters of pg_moisture feed pg_burn directly on the device
+    (the per-cell form of create_terrain_with_fuel_moisture,
+    crates/firelib/src/fuel_moisture.rs:26-63): the GPU chain matches the
+    oracle's chain (oracle moisture -> f32 oracle burn) to the Stage F bar of
+    tests/test_gpu_burn.py.  Slopes in the atanf edge band are left out so
+    that both chains burn identical moisture."""
+    import torch
+    import py_propag as pp
+    from test_gpu_burn import _check
+
+    n = 1 << 16
+    fuel, slope, aspect = _cells(77, n)
+    keep = ~_near_class_edge(slope)
+    fuel, slope, aspect = fuel[keep], slope[keep], aspect[keep]
+    n = fuel.size
+    r = np.random.Generator(np.random.PCG64(78))
+    ws = r.uniform(0.0, 10.0, n).astype(np.float32)
+    waz = np.radians(r.uniform(0.0, 360.0, n)).astype(np.float32)
+    wx = dict(temperature=r.uniform(5, 38, 24), humidity=r.uniform(10, 90, 24), cloud_cover=r.uniform(0, 80, 24),
+              rain_6_days=[0.0, 0.0, 1.5, 0.0, 6.0, 0.0], month=7)
+    hour = 14
+    dev = torch.device("cuda:0")
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    t_fuel, t_slope, t_aspect = put(fuel), put(slope), put(aspect)
+    moist = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
+    pp.moisture_device(t_fuel, t_slope, t_aspect, pp.day_weather(**wx), hour, *moist)
+    out = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    pp.burn_device(t_fuel, *moist, put(ws), put(waz), t_slope, t_aspect, *out)
+    torch.cuda.synchronize()
+    gpu = tuple(o.cpu().numpy() for o in out)
+
+    ow = oracle.day_weather(wx["temperature"], wx["humidity"], wx["cloud_cover"], wx["rain_6_days"], 7)
+    d1, d10, d100, herb, wood = oracle.moisture(fuel, slope, aspect, ow, hour)
+    for got, want in zip(moist, (d1, d10, d100, herb, wood)):
+        assert np.array_equal(got.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    terrain = {"fuel": fuel, "d1hr": d1, "d10hr": d10, "d100hr": d100, "herb": herb, "wood": wood,
+               "wind_speed": ws, "wind_azimuth": waz, "slope": slope, "aspect": aspect}
+    cpu = oracle.burn_f32(terrain, nthreads=16)
+    assert (cpu[0] > 0).sum() > n // 2          # most cells spread
+    _check(gpu, cpu, terrain)
