@@ -96,25 +96,32 @@ class GeoReference:
     pixel_height]; a negative pixel_height is north-up."""
 
     def __init__(self, width: int, height: int, proj: str, transform):
+        # The checks and messages of PyGeoReference::new
+        # (crates/py-propag/src/lib.rs:272-330).
         if width <= 0 or height <= 0:
             raise PropagValidationError("Grid dimensions must be positive (width > 0, height > 0)")
-        if width > 65535 or height > 65535:
-            raise PropagValidationError("Grid dimensions too large (max 65535 per side, "
-                                        "16-bit references)")
+        if width > 32768 or height > 32768:
+            raise PropagValidationError("Grid dimensions too large (max 32768x32768)")
+        if not proj:
+            raise PropagValidationError("Projection string cannot be empty")
         pb = proj.encode()
         if len(pb) >= 1024:
             raise PropagValidationError("Projection string too large (max 1023 characters)")
         t = [float(v) for v in np.asarray(transform, dtype=np.float64).ravel()]
         if len(t) != 6:
-            raise PropagValidationError("Geo transform must have exactly 6 elements")
-        if t[1] == 0.0 or t[5] == 0.0 or not all(math.isfinite(v) for v in t):
+            raise PropagValidationError("Geo transform must have exactly 6 elements [x_origin, pixel_width, "
+                                        "x_rotation, y_origin, y_rotation, pixel_height]")
+        if t[1] == 0.0 or t[5] == 0.0:
             raise PropagValidationError("Invalid geo transform: pixel size cannot be zero")
+        if not all(math.isfinite(float(np.float32(v))) for v in t):
+            raise PropagValidationError("Invalid geo transform parameters - ensure pixel sizes are "
+                                        "non-zero and finite")
         self._width, self._height, self._proj = int(width), int(height), proj
         self._gt = [np.float32(v) for v in t]
         self._inv = _invert(self._gt)
 
     @classmethod
-    def south_up(cls, bbox, pixel_size, proj=""):
+    def south_up(cls, bbox, pixel_size, proj):
         """GeoReference::south_up (crates/geometry/src/lib.rs:239-257)."""
         (x0, y0), (x1, y1) = bbox
         dx, dy = pixel_size
@@ -123,7 +130,7 @@ class GeoReference:
         return cls(w, h, proj, [x0, dx, 0.0, y0, 0.0, dy])
 
     @classmethod
-    def north_up(cls, bbox, pixel_size, proj=""):
+    def north_up(cls, bbox, pixel_size, proj):
         """GeoReference::north_up (crates/geometry/src/lib.rs:258-276)."""
         (x0, y0), (x1, y1) = bbox
         dx, dy = pixel_size
@@ -168,10 +175,15 @@ class Settings:
 
     @max_time.setter
     def max_time(self, v):
-        v = float(v)
-        if not (v > 0 and math.isfinite(v)):
-            raise PropagValidationError("max_time must be positive and finite")
-        self._max_time = np.float32(v)
+        # PySettings::new / set_max_time (crates/py-propag/src/lib.rs:183-233)
+        v = np.float32(v)
+        if v <= 0:
+            raise PropagValidationError("Maximum simulation time must be positive")
+        if not np.isfinite(v):
+            raise PropagValidationError("Maximum simulation time must be a finite number")
+        if v > np.float32(10.0 * 365.0 * 24.0 * 3600.0):
+            raise PropagValidationError("Maximum simulation time too large (limit: 10 years)")
+        self._max_time = v
 
     def to_c(self) -> _native.SettingsC:
         s = _native.SettingsC()
@@ -189,6 +201,21 @@ class TimeFeature:
     time: float
     geom_wkb: bytes = b""
     cells: list = field(default_factory=list)
+
+    def __post_init__(self):
+        # PyTimeFeature::new (crates/py-propag/src/lib.rs:392-418); a feature
+        # given only as raster cells carries no WKB.
+        t = np.float32(self.time)
+        if t < 0:
+            raise PropagValidationError("Time must be non-negative")
+        if not np.isfinite(t):
+            raise PropagValidationError("Time must be a finite number")
+        self.geom_wkb = bytes(self.geom_wkb)
+        if not self.cells:
+            if not self.geom_wkb:
+                raise PropagValidationError("Geometry WKB cannot be empty")
+            if len(self.geom_wkb) < 9:
+                raise PropagValidationError("Invalid geometry WKB - too short for valid geometry")
 
 
 @dataclass
@@ -208,7 +235,11 @@ class PropagResults:
 
 
 def check_cuda_available() -> bool:
-    """True when a gfx950 GPU is visible (name kept from the reference)."""
+    """True when a gfx950 GPU is visible (name kept from the reference,
+    crates/py-propag/src/lib.rs:26-56, which asks nvidia-smi and falls back
+    to the device node): no KFD device node means no GPU."""
+    if not os.path.exists("/dev/kfd"):
+        return False
     try:
         import torch
         return bool(torch.cuda.is_available()) and "gfx950" in torch.cuda.get_device_properties(0).gcnArchName
@@ -556,8 +587,12 @@ def burn(terrain: dict):
     return tuple(o.cpu().numpy() for o in out)
 
 
+# The reference module's class names (crates/py-propag/py_propag/__init__.py).
+PySettings, PyGeoReference, PyTimeFeature, PyPropagResults = Settings, GeoReference, TimeFeature, PropagResults
+
 __all__ = [
-    "Settings", "GeoReference", "TimeFeature", "PropagResults", "propagate", "load_terrain_data",
+    "Settings", "PySettings", "GeoReference", "PyGeoReference", "TimeFeature", "PyTimeFeature",
+    "PropagResults", "PyPropagResults", "propagate", "load_terrain_data",
     "check_cuda_available", "check_gpu_available", "PropagCudaError", "PropagGpuError",
     "PropagGdalError", "PropagValidationError", "PropagMPIError", "PropagNotConverged", "burn",
     "burn_device", "propagate_device", "initial_state", "rasterize_times", "rasterize_fuels",

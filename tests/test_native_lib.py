@@ -53,7 +53,7 @@ def test_georeference_and_settings_validation():
 
     g = pp.GeoReference.south_up(((0.0, 0.0), (5120.0, 5120.0)), (5.0, 5.0), "+proj=utm +zone=30")
     assert (g.width, g.height) == (1024, 1024) and not g.is_north_up
-    gn = pp.GeoReference.north_up(((0.0, 0.0), (50.0, 30.0)), (5.0, 5.0))
+    gn = pp.GeoReference.north_up(((0.0, 0.0), (50.0, 30.0)), (5.0, 5.0), "+proj=utm +zone=30")
     assert (gn.width, gn.height) == (10, 6) and gn.is_north_up
     c = g.to_c()
     assert c.transform.inv.dx == pytest.approx(0.2)
