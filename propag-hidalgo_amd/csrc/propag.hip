@@ -1849,9 +1849,11 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   unsigned book[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned launched_groups = 0;
   PG_HIP(hipEventRecord(evr0, stream));
-  if (tn.mode == 1) {
+  // One launch per sweep, the host reading the bookkeeping in between.
+  auto run_per_sweep = [&]() -> pg_status {
     const dim3 grid(cu * 4);
     launched_groups = grid.x;
+    a.lds_bits = 0;
     for (int k = L.k_begin;; ++k) {
       if (L.cache)
         hipLaunchKernelGGL(propagate_sweep<true>, grid, dim3(256), 0, stream, a, k);
@@ -1864,6 +1866,11 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       if (book[0]) break;
     }
     PG_HIP(hipEventRecord(evr1, stream));
+    return PG_OK;
+  };
+  if (tn.mode == 1) {
+    st = run_per_sweep();
+    if (st != PG_OK) return st;
   } else {
     // Waves per workgroup: as many tile slices as fit beside the bitset when
     // the bitset can live in LDS (one workgroup per CU), else 16 waves.
@@ -1910,21 +1917,45 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       pk.sharedMemBytes = static_cast<unsigned>(shmem);
       pk.kernelParams = kargs;
       PG_HIP(hipGraphExecKernelNodeSetParams(lg->exec, lg->knode, &pk));
-      PG_HIP(hipGraphLaunch(lg->exec, stream));
-      PG_HIP(hipStreamSynchronize(stream));
-      std::memcpy(book, &lg->host->book, sizeof(book));
-      unsigned long long counters[16];
-      std::memcpy(counters, lg->host->stats, sizeof(counters));
-      float gms = 0.0f;
-      PG_HIP(hipEventElapsedTime(&gms, lg->ev0, lg->ev1));
-      return finish_stats(dev, book, counters, gms, launched_groups, a, prof_rows, stats, res);
+      const hipError_t ge = hipGraphLaunch(lg->exec, stream);
+      if (ge != hipErrorIllegalState && ge != hipErrorCooperativeLaunchTooLarge) {
+        PG_HIP(ge);
+        PG_HIP(hipStreamSynchronize(stream));
+        std::memcpy(book, &lg->host->book, sizeof(book));
+        unsigned long long counters[16];
+        std::memcpy(counters, lg->host->stats, sizeof(counters));
+        float gms = 0.0f;
+        PG_HIP(hipEventElapsedTime(&gms, lg->ev0, lg->ev1));
+        return finish_stats(dev, book, counters, gms, launched_groups, a, prof_rows, stats, res);
+      }
+      // Cooperative launch refused (as below): per-sweep launches instead,
+      // after the clear the graph would have made.
+      (void)hipGetLastError();
+      hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
+                         sizeof(Control) / 4, 0u);
+      PG_HIP(hipGetLastError());
+      st = run_per_sweep();
+      if (st != PG_OK) return st;
+    } else {
+      void *kargs[] = {&a};
+      const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups), dim3(block),
+                                                       kargs, static_cast<unsigned>(shmem), stream);
+      if (le == hipErrorIllegalState || le == hipErrorCooperativeLaunchTooLarge) {
+        // The device refuses a cooperative launch to this process while
+        // another process holds its cooperative-launch resources (or when
+        // the grid cannot be co-resident): the same sweeps, one launch each,
+        // give the same results (the schedule does not depend on the launch
+        // form).
+        (void)hipGetLastError();
+        st = run_per_sweep();
+        if (st != PG_OK) return st;
+      } else {
+        PG_HIP(le);
+        PG_HIP(hipEventRecord(evr1, stream));
+        PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
+        PG_HIP(hipStreamSynchronize(stream));
+      }
     }
-    void *kargs[] = {&a};
-    PG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kern), dim3(groups), dim3(block), kargs,
-                                      static_cast<unsigned>(shmem), stream));
-    PG_HIP(hipEventRecord(evr1, stream));
-    PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
-    PG_HIP(hipStreamSynchronize(stream));
   }
   unsigned long long counters[16];
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));

@@ -25,6 +25,19 @@ F = np.float32(np.finfo(np.float32).max)
 
 
 def test_cli_runs_the_readme_scenario(tmp_path):
+    # A propagation in this process first: while a process holds the
+    # device's cooperative-launch resources, the device refuses cooperative
+    # launches to other processes, and the program's propagation must then
+    # run its sweeps as separate launches, with the same results.
+    import py_propag as pp
+    g = pp.GeoReference(64, 64, "+proj=utm +zone=30", [0.0, 5.0, 0.0, 0.0, 0.0, 5.0])
+    terrain = {"fuel": np.ones(64 * 64, np.uint8), **{f: np.full(64 * 64, 0.1, np.float32)
+                                                      for f in ("d1hr", "d10hr", "d100hr", "herb", "wood")},
+               "wind_speed": np.full(64 * 64, 5.0, np.float32), "wind_azimuth": np.zeros(64 * 64, np.float32),
+               "slope": np.zeros(64 * 64, np.float32), "aspect": np.zeros(64 * 64, np.float32)}
+    first = pp.propagate(pp.Settings(g, 3600.0), terrain, [(32, 32, 0.0)])
+    assert (first.arrival_times < F).sum() > 1
+
     out = tmp_path / "tiempos.tif"
     r = subprocess.run([CLI, str(out)], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr
