@@ -501,6 +501,7 @@ struct WalkReq {
   float time;
 };
 constexpr int kWalkQ = kWave;
+static_assert(kWalkQ >= kWave, "a tile defers at most one walk per lane, after at most one flush");
 
 template <bool kCache, bool kLdsBits>
 constexpr bool defer_walks() { return kCache && !kLdsBits; }
