@@ -503,12 +503,26 @@ struct WalkReq {
 constexpr int kWalkQ = kWave;
 static_assert(kWalkQ >= kWave, "a tile defers at most one walk per lane, after at most one flush");
 
+// In the same launches a wave also holds back the list entries of the tiles
+// it queues for the next sweep (their marks are set at once, so each tile
+// is still queued once) and appends them with one atomic add per batch
+// instead of one round trip per changed tile.
+constexpr int kEntryQ = 2 * kWave;
+static_assert(kEntryQ >= kWave, "one round of marks fits after at most one flush");
+
+struct DeferLds {  // one wave's deferred work (LDS)
+  WalkReq walk[kWalkQ];
+  unsigned long long entry[kEntryQ];
+};
+
 template <bool kCache, bool kLdsBits>
 constexpr bool defer_walks() { return kCache && !kLdsBits; }
 
 struct WalkQueue {
-  WalkReq *q;  // this wave's kWalkQ slots (LDS)
-  unsigned n;  // queued (wave-uniform)
+  WalkReq *q;  // this wave's kWalkQ walk slots (LDS)
+  unsigned n;  // queued walks (wave-uniform)
+  unsigned long long *e;  // this wave's kEntryQ list entries for the next sweep (LDS)
+  unsigned ne;            // held entries (wave-uniform)
 };
 
 // Runs the queued walks, one per lane, and writes those cells' states.
@@ -572,6 +586,36 @@ __device__ inline void mark_commit(const TileQueue &q, uint32_t si, int k1, cons
   }
 }
 
+// Appends the wave's held entries to the list of sweep k1: one atomic add
+// for the batch, then the stores.  Called by all lanes of a wave.
+__device__ inline void flush_entries(const TileQueue &q, WalkQueue &wq, int k1, int lane) {
+  if (wq.ne == 0) return;
+  wave_sync();
+  uint32_t base = 0;
+  if (lane == 0) base = add_relaxed(count_of(q, k1), wq.ne);
+  base = __shfl(base, 0);
+  for (unsigned i = static_cast<unsigned>(lane); i < wq.ne; i += kWave) {
+    const size_t slot = static_cast<size_t>(base) + i;
+    if (slot < q.cap) st_u64(&list_of(q, k1)[slot], wq.e[i]);
+  }
+  wq.ne = 0;
+  wave_sync();
+}
+
+// mark_commit for a wave that holds its entries back: the lanes that set
+// their tile's mark first add the tile to the wave's entries.
+__device__ inline void mark_hold(const TileQueue &q, WalkQueue &wq, uint32_t si, int k1, const Mark &mk, int lane) {
+  const bool fresh = mk.valid && (mk.old & mk.bit) == 0u;
+  const unsigned long long m = __ballot(fresh);
+  if (!m) return;
+  const unsigned nf = static_cast<unsigned>(__builtin_popcountll(m));
+  if (wq.ne + nf > static_cast<unsigned>(kEntryQ)) flush_entries(q, wq, k1, lane);
+  if (fresh)
+    wq.e[wq.ne + static_cast<unsigned>(__builtin_popcountll(m & ((1ull << lane) - 1ull)))] =
+        (static_cast<unsigned long long>(si) << 32) | mk.t;
+  wq.ne += nf;
+}
+
 // Single-thread form for set-up kernels.
 __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t si, int k1, int X, int Y) {
   const uint32_t t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
@@ -626,12 +670,16 @@ __device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, int
 }
 
 // Rounds 1.. of the box; returns the number of tiles marked in all rounds.
+template <bool kHold>
 __device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_t si, int k, const NextTiles &nt,
-                                     int lane) {
+                                     WalkQueue &wq, int lane) {
   for (int b0 = kWave - 9; b0 < nt.n; b0 += kWave) {
     const int b = b0 + lane;
     const Mark mk = mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
-    mark_commit(q, si, k + 1, mk, lane);
+    if (kHold)
+      mark_hold(q, wq, si, k + 1, mk, lane);
+    else
+      mark_commit(q, si, k + 1, mk, lane);
   }
   return 9u + static_cast<uint32_t>(nt.n);
 }
@@ -942,8 +990,11 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   }
   uint64_t tp2 = a.prof ? wall_clock64() : 0;
   if (changed) {
-    mark_commit(a.q, si, k + 1, mk0, lane);
-    const uint32_t marks = next_rest(a.q, sc, si, k, nt, lane);
+    if (kDefer)
+      mark_hold(a.q, wq, si, k + 1, mk0, lane);
+    else
+      mark_commit(a.q, si, k + 1, mk0, lane);
+    const uint32_t marks = next_rest<kDefer>(a.q, sc, si, k, nt, wq, lane);
     if (lane == 0) cnt.marks += marks;
     // Register this tile as a dependent of each new far reference's tile
     // and mark the reference cells, one set of atomics per distinct
@@ -1023,9 +1074,9 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
 // the current tile runs.  Entries fill slots from 0, so a wave stops at the
 // first empty slot it draws.
 template <bool kCache, bool kLdsBits>
-__device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds &s, WalkReq *walkq,
+__device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds &s, DeferLds *dq,
                                Counters &cnt) {
-  WalkQueue wq{walkq, 0u};
+  WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
   unsigned long long *list = list_of(a.q, k);
   auto draw = [&](size_t &e) {
@@ -1050,6 +1101,7 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
     const uint64_t tw = a.prof ? wall_clock64() : 0;
     flush_walks(a, k, wq, cnt);
     if (a.prof) cnt.ph[2] += wall_clock64() - tw;
+    flush_entries(a.q, wq, k + 1, static_cast<int>(lane));
   }
   return changed;
 }
@@ -1062,8 +1114,8 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
 // this wave changed.
 template <bool kCache, bool kLdsBits>
 __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds &s,
-                           WalkReq *walkq, Counters &cnt) {
-  WalkQueue wq{walkq, 0u};
+                           DeferLds *dq, Counters &cnt) {
+  WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
   unsigned long long *list = list_of(a.q, k);
   bool changed = false;
@@ -1084,6 +1136,7 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
     const uint64_t tw = a.prof ? wall_clock64() : 0;
     flush_walks(a, k, wq, cnt);
     if (a.prof) cnt.ph[2] += wall_clock64() - tw;
+    flush_entries(a.q, wq, k + 1, static_cast<int>(lane));
   }
   return changed;
 }
@@ -1264,12 +1317,12 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   const unsigned lane = threadIdx.x & (kWave - 1);
   // Dynamic LDS: (LDS bitset) the boundary bitset at offset 0, so a walk's
   // word address needs no base; then nw tile slices and (deferred walks) nw
-  // walk queues.
+  // blocks of deferred work (walk queue and held list entries).
   const unsigned bits_words = kLdsBits ? (a.lds_words + 3u) & ~3u : 0u;  // 16-byte aligned slices
   TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds + bits_words);
   TileLds &s = slices[wave];
   constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
-  WalkReq *walkq = reinterpret_cast<WalkReq *>(slices + nw) + wave * kWalkQ;
+  DeferLds *dq = reinterpret_cast<DeferLds *>(slices + nw) + wave;
   const unsigned lds_off = 0;
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
   if (threadIdx.x < 2) s_next[threadIdx.x] = 0;
@@ -1281,7 +1334,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
-    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, walkq, cnt);
+    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, dq, cnt);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
@@ -1309,7 +1362,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
 template <bool kCache>
 __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   __shared__ TileLds slices[256 / kWave];
-  __shared__ WalkReq walkq[256 / kWave][kWalkQ];
+  __shared__ DeferLds dq[256 / kWave];
   __shared__ unsigned s_changed;
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
@@ -1317,7 +1370,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
-  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], walkq[wave], cnt);
+  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], &dq[wave], cnt);
   if ((threadIdx.x & (kWave - 1)) == 0 && changed) s_changed = 1u;
   __syncthreads();
   if (threadIdx.x == 0 && s_changed) add_relaxed(&a.ctl->changes.v[k & 1], 1u);
@@ -1885,9 +1938,9 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       a.lds_words = static_cast<unsigned>(L.lds_bits_words);
       waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / sizeof(TileLds));
     }
-    // per-wave LDS: the tile slice, plus a walk queue when walks are deferred
-    // (line-of-sight flags cached and the bitset read from L2)
-    const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(WalkReq) * kWalkQ : 0);
+    // per-wave LDS: the tile slice, plus the deferred-work block when walks
+    // are deferred (line-of-sight flags cached and the bitset read from L2)
+    const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(DeferLds) : 0);
     const int max_waves = (a.lds_bits ? persistent_threads<true>() : persistent_threads<false>()) / kWave;
     if (tn.waves_per_group) {
       // with the bitset in LDS, at most the slices that fit beside it
