@@ -66,7 +66,6 @@ struct alignas(128) Line {
 
 // Grid-wide state of one launch; each hot word sits on its own line.
 struct Control {
-  Line gen;          // [0] barrier release: sweep + 1, bit 31 = finished
   Line book;         // [1] done, [3] error bits, [4] last sweep run
   Line changes;      // [0..1] tiles changed in sweep k (k & 1), plain form only
   unsigned long long stats[16];  // relaxations, improvements, dda_steps, tiles, work, wait,
@@ -440,7 +439,9 @@ struct Args {
   int nscen;
   int k_begin, k_end;
   Control *ctl;
-  Slot *slots;               // grid-barrier arrival lines, one per workgroup
+  Slot *slots;               // per-workgroup profiling lines of the grid barrier
+  unsigned long long *arrive;  // grid-barrier arrival words: [2][arrive_stride]
+  unsigned arrive_stride;
   unsigned spin_limit;
   int lds_bits;              // bitset of scenario 0 staged in LDS
   const uint32_t *lds_src;   // that bitset (row-major)
@@ -463,26 +464,30 @@ struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
 static_assert(sizeof(TileLds) % 8 == 0, "TileLds must keep 8-byte alignment");
 
 
-// Per-workgroup arrival slot of the grid barrier, one 128-byte line each.
+// Per-workgroup profiling line of the grid barrier, 128 bytes each.
 struct alignas(128) Slot {
-  unsigned long long arrive;
-  unsigned long long work;
   unsigned long long ph[4];  // profiling: slowest wave's phase ticks
   unsigned long long t_seen, t_arrive;  // profiling: sweep start / arrival
-  unsigned pad[16];
+  unsigned pad[20];
 };
 static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
 
+// The barrier's workspace: the profiling lines, then the arrival words of
+// both sweep parities.
+constexpr size_t slot_bytes(size_t max_groups) {
+  return max_groups * sizeof(Slot) + 2 * max_groups * sizeof(unsigned long long);
+}
+
 // Profile columns per sweep: slowest wave per phase (staging, update rule,
-// line-of-sight walks, commit and marks), then the barrier: release of the
-// previous sweep -> last workgroup starting, the longest workgroup span,
-// last arrival -> master has seen all, master bookkeeping.
+// line-of-sight walks, commit and marks), then the barrier: workgroup 0's
+// books of the previous sweep done -> last workgroup starting, the longest
+// workgroup span, last arrival -> workgroup 0 has seen all, its books.
 constexpr int kProfCols = 8;
 
 struct Counters {
   uint32_t relax, improve, steps, tiles, marks, idle;
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
-  uint64_t crit_ticks;              // the barrier master: slowest workgroup's work per sweep, summed
+  uint64_t crit_ticks;              // workgroup 0: slowest workgroup's work per sweep, summed
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
   uint64_t ph_total[4];             // profiling: this wave's phase ticks, all sweeps
 };
@@ -1142,21 +1147,26 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
 }
 
 // Grid barrier.  Callers have waited for their stores (wait_stores) and
-// passed __syncthreads.  Every workgroup publishes one 64-bit arrival word in
-// its own line: (sweep + 1) << 32 | changed << 31 | work ticks.  Wave 0 of
-// workgroup 0 polls all lines, a batch of loads at a time, folds the
-// quiet-sweep test, the end of the launch and the critical-path time, and
-// releases the generation word, (sweep + 1) | finished << 31, which thread 0
-// of every other workgroup polls.  All words are device-coherent, so no
-// cache maintenance is needed.  Returns 1 when the launch is finished.
+// passed __syncthreads.  Every workgroup publishes one 64-bit arrival word,
+// (sweep + 1) << 32 | changed << 31 | work ticks, in an array by sweep
+// parity (a workgroup that has passed this barrier writes its next arrival
+// into the other array, so a slower reader still finds this sweep's word).
+// Wave 0 of every workgroup polls all words, a batch of loads at a time and
+// only those not yet seen, and folds the quiet-sweep test and the end of the
+// launch itself: no release word, so a sweep starts one memory round trip
+// after the last arrival.  Workgroup 0 also keeps the books (critical-path
+// time, the list counter reset, the launch outcome).  All words are
+// device-coherent, so no cache maintenance is needed.  Returns 1 when the
+// launch is finished.
 __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t sweep_t0, Counters &cnt,
                             const unsigned long long *wg_ph) {
   __shared__ int s_done;
-  __shared__ unsigned long long s_release;  // master: time of the last release
+  __shared__ unsigned long long s_release;  // workgroup 0: time its books were done
   const unsigned target = static_cast<unsigned>(k - a.k_begin) + 1u;
   const unsigned row = static_cast<unsigned>(k - a.k_begin);
   Slot *slots = a.slots;
   Control *c = a.ctl;
+  unsigned long long *arrive = a.arrive + (target & 1u) * static_cast<size_t>(a.arrive_stride);
   uint64_t t_arrive = 0;
   if (threadIdx.x == 0) {
     t_arrive = wall_clock64();
@@ -1169,57 +1179,58 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
       wait_stores();
     }
     const uint64_t w31 = work < 0x7fffffffull ? work : 0x7fffffffull;
-    st_u64(&slots[blockIdx.x].arrive,
+    st_u64(&arrive[blockIdx.x],
            (static_cast<unsigned long long>(target) << 32) | (wg_changed ? (1ull << 31) : 0ull) | w31);
   }
-  if (blockIdx.x == 0) {
-    if (threadIdx.x < kWave) {
-      const unsigned lane = threadIdx.x;
-      const unsigned G = gridDim.x;
-      const unsigned nq = (G + kWave - 1) / kWave;  // slots per lane (at most 64)
-      unsigned long long pending = 0;
-      for (unsigned q = 0; q < nq; ++q)
-        if (lane + q * kWave < G) pending |= 1ull << q;
-      bool changed = false, timeout = false;
-      unsigned long long maxwork = 0;
-      unsigned spins = 0;
-      while (pending) {
-        for (unsigned q0 = 0; q0 < nq; q0 += 4) {
-          unsigned long long v[4];
+  if (threadIdx.x < kWave) {
+    const unsigned lane = threadIdx.x;
+    const unsigned G = gridDim.x;
+    const unsigned nq = (G + kWave - 1) / kWave;  // words per lane (at most 64)
+    unsigned long long pending = 0;
+    for (unsigned q = 0; q < nq; ++q)
+      if (lane + q * kWave < G) pending |= 1ull << q;
+    bool changed = false, timeout = false;
+    unsigned long long maxwork = 0;
+    unsigned spins = 0;
+    while (pending) {
+      for (unsigned q0 = 0; q0 < nq; q0 += 4) {
+        unsigned long long v[4];
 #pragma unroll
-          for (unsigned j = 0; j < 4; ++j) {
-            const unsigned q = q0 + j;
-            v[j] = (q < nq && ((pending >> q) & 1ull)) ? ld_u64(&slots[lane + q * kWave].arrive) : 0ull;
-          }
-#pragma unroll
-          for (unsigned j = 0; j < 4; ++j) {
-            const unsigned q = q0 + j;
-            if (q < nq && ((pending >> q) & 1ull) && (v[j] >> 32) == target) {
-              pending &= ~(1ull << q);
-              changed = changed || ((v[j] >> 31) & 1ull);
-              const unsigned long long w = v[j] & 0x7fffffffull;
-              maxwork = w > maxwork ? w : maxwork;
-            }
-          }
+        for (unsigned j = 0; j < 4; ++j) {
+          const unsigned q = q0 + j;
+          v[j] = (q < nq && ((pending >> q) & 1ull)) ? ld_u64(&arrive[lane + q * kWave]) : 0ull;
         }
-        if (pending) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > a.spin_limit) {
-            timeout = true;
-            break;
+#pragma unroll
+        for (unsigned j = 0; j < 4; ++j) {
+          const unsigned q = q0 + j;
+          if (q < nq && ((pending >> q) & 1ull) && (v[j] >> 32) == target) {
+            pending &= ~(1ull << q);
+            changed = changed || ((v[j] >> 31) & 1ull);
+            const unsigned long long w = v[j] & 0x7fffffffull;
+            maxwork = w > maxwork ? w : maxwork;
           }
         }
       }
-      asm volatile("" ::: "memory");
+      if (pending) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > a.spin_limit) {
+          timeout = true;
+          break;
+        }
+      }
+    }
+    asm volatile("" ::: "memory");
+    changed = __ballot(changed) != 0ull;
+    timeout = __ballot(timeout) != 0ull;
+    const bool done = !changed || k + 1 >= a.k_end || timeout;
+    if (blockIdx.x == 0) {
       for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(maxwork, off);
         maxwork = o > maxwork ? o : maxwork;
       }
-      changed = __ballot(changed) != 0ull;
-      timeout = __ballot(timeout) != 0ull;
       if (a.prof && row < a.prof_sweeps) {
         const unsigned long long t_all = wall_clock64();
-        // first sweep: the kernel start of workgroup 0 stands in for a release
+        // first sweep: the kernel start of workgroup 0 stands in for the books
         const unsigned long long rel = row == 0 ? __shfl(sweep_t0, 0) : s_release;
         unsigned long long m[7] = {0, 0, 0, 0, 0, 0, 0};  // ph[4], start, span, arrival
         for (unsigned i = lane; i < G; i += kWave) {
@@ -1235,12 +1246,10 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
           unsigned long long *r = a.prof + static_cast<size_t>(row) * kProfCols;
           for (int q = 0; q < 6; ++q) r[q] = m[q];
           r[6] = t_all > m[6] ? t_all - m[6] : 0ull;
-          r[7] = t_all;  // completed below with the release time
+          r[7] = t_all;  // completed below with the end of the books
         }
       }
       if (lane == 0) {
-        const bool done = !changed || k + 1 >= a.k_end || timeout;
-        if (timeout) st_relaxed(&c->book.v[3], 1u);
         if (done) {
           st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
           st_relaxed(&c->book.v[1], changed ? 0u : 1u);
@@ -1256,23 +1265,12 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
             r[7] = s_release - r[7];
           }
         }
-        st_relaxed(&c->gen.v[0], target | (done ? 0x80000000u : 0u));
-        s_done = done ? 1 : 0;
       }
     }
-  } else if (threadIdx.x == 0) {
-    unsigned spins = 0, g;
-    bool timeout = false;
-    while (((g = ld_relaxed(&c->gen.v[0])) & 0x7fffffffu) != target) {
-      __builtin_amdgcn_s_sleep(1);
-      if (++spins > a.spin_limit) {
-        timeout = true;
-        st_relaxed(&c->book.v[3], 1u);
-        break;
-      }
+    if (lane == 0) {
+      if (timeout) st_relaxed(&c->book.v[3], 1u);
+      s_done = done ? 1 : 0;
     }
-    asm volatile("" ::: "memory");
-    s_done = (timeout || (g >> 31) != 0u) ? 1 : 0;
   }
   __syncthreads();
   if (threadIdx.x == 0) cnt.wait_ticks += wall_clock64() - t_arrive;
@@ -1633,7 +1631,7 @@ pg_status build_graph(LaunchGraph *lg, const Args &a, size_t max_groups) {
   PG_HIP(hipHostMalloc(reinterpret_cast<void **>(&lg->host), sizeof(Control), hipHostMallocDefault));
   uint32_t *ctl = reinterpret_cast<uint32_t *>(a.ctl);
   uint32_t *slots = reinterpret_cast<uint32_t *>(a.slots);
-  size_t ctl_words = sizeof(Control) / 4, slot_words = max_groups * sizeof(Slot) / 4;
+  size_t ctl_words = sizeof(Control) / 4, slot_words = slot_bytes(max_groups) / 4;
   uint32_t zero = 0;
   void *args_ctl[] = {&ctl, &ctl_words, &zero};
   void *args_slots[] = {&slots, &slot_words, &zero};
@@ -1851,7 +1849,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   const unsigned prof_rows = tn.profile ? kProfRows : 0u;
   size_t off = 0;
   const size_t o_ctl = off; off = align_up(off + sizeof(Control), 256);
-  const size_t o_slots = off; off = align_up(off + max_groups * sizeof(Slot), 256);
+  const size_t o_slots = off; off = align_up(off + slot_bytes(max_groups), 256);
   const size_t o_prof = off; off = align_up(off + static_cast<size_t>(kProfRows) * kProfCols * 8, 256);
   pg_status st = ensure(&pd->launch, off);
   if (st != PG_OK) return st;
@@ -1866,6 +1864,8 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   a.k_end = L.k_end;
   a.ctl = reinterpret_cast<Control *>(base + o_ctl);
   a.slots = reinterpret_cast<Slot *>(base + o_slots);
+  a.arrive = reinterpret_cast<unsigned long long *>(a.slots + max_groups);
+  a.arrive_stride = static_cast<unsigned>(max_groups);
   a.spin_limit = 1u << (tn.spin_limit_log2 ? (tn.spin_limit_log2 > 30 ? 30 : tn.spin_limit_log2) : 22);
   a.prof = prof_rows ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
   a.prof_sweeps = prof_rows;
@@ -1881,8 +1881,8 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   if (!use_graph) {
     hipLaunchKernelGGL(fill_u32, dim3(1), dim3(256), 0, stream, reinterpret_cast<uint32_t *>(a.ctl),
                        sizeof(Control) / 4, 0u);
-    hipLaunchKernelGGL(fill_u32, grid_for(max_groups * sizeof(Slot) / 4), dim3(256), 0, stream,
-                       reinterpret_cast<uint32_t *>(a.slots), max_groups * sizeof(Slot) / 4, 0u);
+    hipLaunchKernelGGL(fill_u32, grid_for(slot_bytes(max_groups) / 4), dim3(256), 0, stream,
+                       reinterpret_cast<uint32_t *>(a.slots), slot_bytes(max_groups) / 4, 0u);
   }
   if (prof_rows)
     hipLaunchKernelGGL(fill_u32, grid_for(static_cast<size_t>(prof_rows) * kProfCols * 2), dim3(256), 0, stream,
@@ -1956,7 +1956,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (per_cu < 1) return set_error(PG_ERR_INVALID, "persistent kernel does not fit on a CU");
     unsigned groups = cu * static_cast<unsigned>(per_cu);
     if (groups > max_groups) groups = static_cast<unsigned>(max_groups);
-    if (groups > 64u * kWave) groups = 64u * kWave;  // the barrier master polls <= 64 slots per lane
+    if (groups > 64u * kWave) groups = 64u * kWave;  // a barrier poller reads <= 64 words per lane
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
     launched_groups = groups;
     if (use_graph) {
