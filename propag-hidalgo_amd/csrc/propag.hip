@@ -83,9 +83,6 @@ __device__ inline PG_GLOBAL T *G(T *p) {
   return (PG_GLOBAL T *)(p);
 }
 
-__device__ inline unsigned ld_relaxed(const unsigned *p) {
-  return __hip_atomic_load(G(const_cast<unsigned *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ inline void st_relaxed(unsigned *p, unsigned v) {
   __hip_atomic_store(G(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1319,7 +1316,6 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   const unsigned bits_words = kLdsBits ? (a.lds_words + 3u) & ~3u : 0u;  // 16-byte aligned slices
   TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds + bits_words);
   TileLds &s = slices[wave];
-  constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
   DeferLds *dq = reinterpret_cast<DeferLds *>(slices + nw) + wave;
   const unsigned lds_off = 0;
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
