@@ -161,6 +161,25 @@ pg_status pg_propagate(const Settings *settings, const float *speed_max,
                        uint32_t *boundaries, const pg_tuning *tuning,
                        pg_stats *stats, void *stream);
 
+/* Allocation-free form of pg_propagate, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  It has no counterpart in the reference.  Same
+ * arguments, results and errors as pg_propagate, but every buffer beyond the
+ * caller's inputs and outputs comes from a fixed device array of the library
+ * (40 MiB, held on each device the library runs on until the process ends),
+ * and the call makes no hipMalloc/hipFree, creates no events and caches no
+ * graph.  Limits: at most PG_NOALLOC_MAX_CELLS cells (larger rasters return
+ * PG_ERR_INVALID), tuning->mode 0 or 1 (2 and profile = 1 return
+ * PG_ERR_INVALID), stats->elapsed_ms and relax_ms are reported as 0.  Calls
+ * on one device share the array: they are serialised, and each synchronises
+ * its stream before returning. */
+#define PG_NOALLOC_MAX_CELLS 1048576u
+pg_status pg_propagate_noalloc(const Settings *settings, const float *speed_max,
+                               const float *azimuth_max, const float *eccentricity,
+                               float *time, uint16_t *refs_x, uint16_t *refs_y,
+                               uint32_t *boundaries, const pg_tuning *tuning,
+                               pg_stats *stats, void *stream);
+
 /* Host-buffer form of the whole device data plane of propagate()
  * (crates/propag/src/propag.rs:375-573): uploads the SoA terrain, runs the
  * burn pass then pg_propagate, copies time / refs / boundaries back.  The

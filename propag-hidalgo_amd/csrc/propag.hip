@@ -1830,6 +1830,14 @@ static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const
                               float rms, unsigned launched_groups, const Args &a, unsigned prof_rows,
                               pg_stats *stats, LaunchResult *res);
 
+size_t launch_ws_bytes(unsigned cu, bool profile) {
+  const size_t max_groups = static_cast<size_t>(cu) * 32;
+  size_t off = align_up(sizeof(Control), 256);
+  off = align_up(off + slot_bytes(max_groups), 256);
+  if (profile) off = align_up(off + static_cast<size_t>(kProfRows) * kProfCols * 8, 256);
+  return off;
+}
+
 pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tuning, pg_stats *stats,
                      LaunchResult *res, hipStream_t stream) {
   pg_tuning tn{};
@@ -1837,19 +1845,26 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 8");
   if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 8");
   if (L.k_end <= L.k_begin) return set_error(PG_ERR_INVALID, "empty sweep range");
-  PerDevice *pd = per_device(dev->device);
-  std::lock_guard<std::mutex> lk(pd->launch_mu);
+  // A caller's workspace has no room for profile rows and no cached graphs.
+  if (L.ws && (tn.profile || tn.mode == 2))
+    return set_error(PG_ERR_INVALID, "profiling and the graph launch form need the library's own workspace");
+  PerDevice *pd = L.ws ? nullptr : per_device(dev->device);
+  std::unique_lock<std::mutex> lk;
+  if (pd) lk = std::unique_lock<std::mutex>(pd->launch_mu);
 
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
   const size_t max_groups = static_cast<size_t>(cu) * 32;
   const unsigned prof_rows = tn.profile ? kProfRows : 0u;
-  size_t off = 0;
-  const size_t o_ctl = off; off = align_up(off + sizeof(Control), 256);
-  const size_t o_slots = off; off = align_up(off + slot_bytes(max_groups), 256);
-  const size_t o_prof = off; off = align_up(off + static_cast<size_t>(kProfRows) * kProfCols * 8, 256);
-  pg_status st = ensure(&pd->launch, off);
-  if (st != PG_OK) return st;
-  char *base = static_cast<char *>(pd->launch.base);
+  const size_t o_ctl = 0;
+  const size_t o_slots = align_up(sizeof(Control), 256);
+  const size_t o_prof = align_up(o_slots + slot_bytes(max_groups), 256);
+  pg_status st = PG_OK;
+  char *base = L.ws;
+  if (!base) {
+    st = ensure(&pd->launch, launch_ws_bytes(cu, true));
+    if (st != PG_OK) return st;
+    base = static_cast<char *>(pd->launch.base);
+  }
 
   Args a{};
   a.scen = L.scen;
@@ -1886,8 +1901,10 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   PG_HIP(hipGetLastError());
 
   hipEvent_t evr0 = nullptr, evr1 = nullptr;
-  PG_HIP(hipEventCreate(&evr0));
-  PG_HIP(hipEventCreate(&evr1));
+  if (!L.untimed) {
+    PG_HIP(hipEventCreate(&evr0));
+    PG_HIP(hipEventCreate(&evr1));
+  }
   struct EventGuard {
     hipEvent_t *e[2];
     ~EventGuard() {
@@ -1898,7 +1915,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
 
   unsigned book[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned launched_groups = 0;
-  PG_HIP(hipEventRecord(evr0, stream));
+  if (evr0) PG_HIP(hipEventRecord(evr0, stream));
   // One launch per sweep, the host reading the bookkeeping in between.
   auto run_per_sweep = [&]() -> pg_status {
     const dim3 grid(cu * 4);
@@ -1915,7 +1932,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       PG_HIP(hipStreamSynchronize(stream));
       if (book[0]) break;
     }
-    PG_HIP(hipEventRecord(evr1, stream));
+    if (evr1) PG_HIP(hipEventRecord(evr1, stream));
     return PG_OK;
   };
   if (tn.mode == 1) {
@@ -2001,7 +2018,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
         if (st != PG_OK) return st;
       } else {
         PG_HIP(le);
-        PG_HIP(hipEventRecord(evr1, stream));
+        if (evr1) PG_HIP(hipEventRecord(evr1, stream));
         PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
         PG_HIP(hipStreamSynchronize(stream));
       }
@@ -2011,7 +2028,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
   float rms = 0.0f;
-  PG_HIP(hipEventElapsedTime(&rms, evr0, evr1));
+  if (evr0) PG_HIP(hipEventElapsedTime(&rms, evr0, evr1));
   return finish_stats(dev, book, counters, rms, launched_groups, a, prof_rows, stats, res);
 }
 
@@ -2056,11 +2073,37 @@ static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const
   return PG_OK;
 }
 
+// Where one propagation call keeps its buffers: the scenario rasters, tile
+// queues and descriptors (`call`, `call_bytes` long) and the launch's control
+// block (`launch`, launch_ws_bytes(cu, false) long).  Null: the per-device
+// workspaces, grown with hipMalloc and held for the call.
+struct CallSpace {
+  char *call;
+  size_t call_bytes;
+  char *launch;
+};
+
+// Bytes of the call workspace of one scenario on the raster of sc.
+static size_t call_ws_bytes(const Scen &sc) {
+  const size_t W = static_cast<size_t>(sc.g.W), H = static_cast<size_t>(sc.g.H);
+  const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
+  const size_t nwords = size_t{1} << sc.nwords_log2;
+  size_t off = align_up(scen_buffer_bytes(W, H), 256);
+  off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
+  off = align_up(off + 4 * ntiles * sizeof(unsigned long long), 256);
+  off = align_up(off + 512, 256);
+  off = align_up(off + sizeof(Scen), 256);
+  off = align_up(off + sizeof(ScenOut), 256);
+  return off;
+}
+
 // The one-scenario entry point: prepare, run to convergence (or the cap),
-// finish.  Buffers come from a per-device workspace held for the call.
-pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const float *speed, const float *az,
-                        const float *ecc, float *time, uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries,
-                        const pg_tuning *tuning, pg_stats *stats, hipStream_t stream) {
+// finish.  Buffers come from `space`, or from a per-device workspace held for
+// the call when `space` is null (then the call is also timed with HIP events).
+static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings, const float *speed,
+                                const float *az, const float *ecc, float *time, uint16_t *refs_x, uint16_t *refs_y,
+                                uint32_t *boundaries, const pg_tuning *tuning, pg_stats *stats, hipStream_t stream,
+                                const CallSpace *space) {
   Scen sc;
   pg_status st = scen_geometry(settings, &sc);
   if (st != PG_OK) return st;
@@ -2070,22 +2113,29 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   if (tuning) tn = *tuning;
   if (tn.max_iterations) sc.max_iters = tn.max_iterations;
   if (sc.max_iters > 0x3fffffffu) return set_error(PG_ERR_INVALID, "max_iterations too large");
-  PerDevice *pd = per_device(dev->device);
-  std::lock_guard<std::mutex> lk(pd->call_mu);
 
   const size_t W = static_cast<size_t>(sc.g.W), H = static_cast<size_t>(sc.g.H);
   const uint32_t nwords = 1u << sc.nwords_log2;
-  size_t off = 0;
-  const size_t o_scen = off; off = align_up(off + scen_buffer_bytes(W, H), 256);
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
-  const size_t o_mark = off; off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
-  const size_t o_list = off; off = align_up(off + 4 * ntiles * sizeof(unsigned long long), 256);
-  const size_t o_count = off; off = align_up(off + 512, 256);
-  const size_t o_desc = off; off = align_up(off + sizeof(Scen), 256);
-  const size_t o_out = off; off = align_up(off + sizeof(ScenOut), 256);
-  st = ensure(&pd->call, off);
-  if (st != PG_OK) return st;
-  char *base = static_cast<char *>(pd->call.base);
+  const size_t o_scen = 0;
+  const size_t o_mark = align_up(o_scen + scen_buffer_bytes(W, H), 256);
+  const size_t o_list = align_up(o_mark + 2 * nwords * sizeof(uint32_t), 256);
+  const size_t o_count = align_up(o_list + 4 * ntiles * sizeof(unsigned long long), 256);
+  const size_t o_desc = align_up(o_count + 512, 256);
+  const size_t o_out = align_up(o_desc + sizeof(Scen), 256);
+  const size_t need = call_ws_bytes(sc);
+  char *base = nullptr;
+  std::unique_lock<std::mutex> lk;
+  if (space) {
+    if (need > space->call_bytes) return set_error(PG_ERR_INVALID, "raster too large for the workspace");
+    base = space->call;
+  } else {
+    PerDevice *pd = per_device(dev->device);
+    lk = std::unique_lock<std::mutex>(pd->call_mu);
+    st = ensure(&pd->call, need);
+    if (st != PG_OK) return st;
+    base = static_cast<char *>(pd->call.base);
+  }
   const ScenBuffers b = scen_carve(base + o_scen, W, H);
   sc.coef = b.coef;
   sc.st[0] = b.st0;
@@ -2105,9 +2155,12 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   Scen *d_scen = reinterpret_cast<Scen *>(base + o_desc);
   ScenOut *d_out = reinterpret_cast<ScenOut *>(base + o_out);
 
+  const bool timed = space == nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  PG_HIP(hipEventCreate(&ev0));
-  PG_HIP(hipEventCreate(&ev1));
+  if (timed) {
+    PG_HIP(hipEventCreate(&ev0));
+    PG_HIP(hipEventCreate(&ev1));
+  }
   struct EventGuard {
     hipEvent_t *e[2];
     ~EventGuard() {
@@ -2115,7 +2168,7 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
         if (*p) (void)hipEventDestroy(*p);
     }
   } guard{{&ev0, &ev1}};
-  PG_HIP(hipEventRecord(ev0, stream));
+  if (ev0) PG_HIP(hipEventRecord(ev0, stream));
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
   const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * ntiles + 255) / 256, cu * 8));
   hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, sc.mark[0], static_cast<size_t>(2 * nwords), 0u);
@@ -2137,19 +2190,25 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   L.cache = sc.cache;
   L.lds_bits_src = boundaries;
   L.lds_bits_words = (W * H + 31) / 32;
+  L.ws = space ? space->launch : nullptr;
+  L.untimed = timed ? 0 : 1;
   pg_stats local{};
   LaunchResult res{};
   st = run_sweeps(dev, L, &tn, &local, &res, stream);
-  if (st != PG_OK) return st;
+  if (st != PG_OK) {
+    // Leave no work of this call queued on the workspace.
+    (void)hipStreamSynchronize(stream);
+    return st;
+  }
   ScenOut o{};
   PG_HIP(hipMemcpyAsync(&o, d_out, sizeof(o), hipMemcpyDeviceToHost, stream));
   PG_HIP(hipStreamSynchronize(stream));
   st = scen_finish(sc, scen_final_buf(sc, o), speed, time, refs_x, refs_y, stream);
   if (st != PG_OK) return st;
-  PG_HIP(hipEventRecord(ev1, stream));
+  if (ev1) PG_HIP(hipEventRecord(ev1, stream));
   PG_HIP(hipStreamSynchronize(stream));
   float ms = 0.0f;
-  PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+  if (ev0) PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
   if (stats) {
     *stats = local;
     stats->iterations = o.capped ? sc.max_iters : static_cast<uint64_t>(o.last_change - sc.k0 + 2);
@@ -2159,4 +2218,95 @@ pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const f
   return PG_OK;
 }
 
+pg_status run_propagate(const DeviceInfo *dev, const Settings *settings, const float *speed, const float *az,
+                        const float *ecc, float *time, uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries,
+                        const pg_tuning *tuning, pg_stats *stats, hipStream_t stream) {
+  return propagate_call(dev, settings, speed, az, ecc, time, refs_x, refs_y, boundaries, tuning, stats, stream,
+                        nullptr);
+}
+
 }  // namespace pg
+
+// ------------------------------------------- allocation-free entry point
+//
+// pg_propagate_noalloc runs pg_propagate's kernels with every buffer beyond
+// the caller's inputs and outputs carved from `scratch`, a device array of
+// this code object: each device the library runs on holds its own copy
+// (kNoAllocScratchBytes) until the process ends.  Its address on a device is
+// taken once, with that device current, by hipGetSymbolAddress; host code
+// passes only that address to kernels and copies (the array's name is a
+// host-side stand-in, not a device address).  Calls on one device are
+// serialised by a lock and each one synchronises its stream before it
+// returns; a call clears what must start at zero and leaves its contents
+// behind for the next.
+static __device__ __align__(256) unsigned char scratch[pg::kNoAllocScratchBytes];
+
+namespace pg {
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+  std::mutex mu;        // held for a whole call
+  bool ready = false;   // set-up done
+  char *ptr = nullptr;  // this device's `scratch`
+  DeviceInfo info{};    // properties (no catalog: the propagation does not burn)
+};
+
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+// Set-up on the current device (under d.mu): device properties and the
+// address of this device's `scratch`.
+pg_status noalloc_setup(NoAllocDevice &d, int dev) {
+  hipDeviceProp_t prop;
+  PG_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_error(PG_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                           "; this library is built for gfx950 only");
+  int lds = 0;
+  PG_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    (void)hipGetLastError();
+    return set_error(PG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
+  d.info.device = dev;
+  d.info.cu_count = prop.multiProcessorCount;
+  d.info.lds_per_block = static_cast<size_t>(lds);
+  d.info.catalog = nullptr;
+  d.ptr = static_cast<char *>(p);
+  d.ready = true;
+  return PG_OK;
+}
+
+}  // namespace
+}  // namespace pg
+
+extern "C" pg_status pg_propagate_noalloc(const Settings *settings, const float *speed_max,
+                                          const float *azimuth_max, const float *eccentricity, float *time,
+                                          uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries,
+                                          const pg_tuning *tuning, pg_stats *stats, void *stream) {
+  using namespace pg;
+  if (!settings) return set_error(PG_ERR_INVALID, "settings is NULL");
+  const uint64_t cells = static_cast<uint64_t>(settings->geo_ref.width) * settings->geo_ref.height;
+  if (cells > PG_NOALLOC_MAX_CELLS)
+    return set_error(PG_ERR_INVALID, "pg_propagate_noalloc: raster of " + std::to_string(cells) +
+                                         " cells exceeds its maximum of " + std::to_string(PG_NOALLOC_MAX_CELLS));
+  if (tuning && (tuning->mode == 2 || tuning->profile))
+    return set_error(PG_ERR_INVALID, "pg_propagate_noalloc: mode 2 (graph) and profiling allocate; use mode 0 or 1");
+  int dev = 0;
+  PG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return set_error(PG_ERR_NO_DEVICE, "device ordinal out of range");
+  NoAllocDevice &d = g_noalloc[dev];
+  std::lock_guard<std::mutex> lk(d.mu);
+  if (!d.ready) {
+    const pg_status s = noalloc_setup(d, dev);
+    if (s != PG_OK) return s;
+  }
+  const size_t launch_bytes = launch_ws_bytes(static_cast<unsigned>(d.info.cu_count), false);
+  if (launch_bytes >= kNoAllocScratchBytes) return set_error(PG_ERR_INVALID, "device too large for the scratch array");
+  const CallSpace space{d.ptr + launch_bytes, kNoAllocScratchBytes - launch_bytes, d.ptr};
+  return propagate_call(&d.info, settings, speed_max, azimuth_max, eccentricity, time, refs_x, refs_y, boundaries,
+                        tuning, stats, static_cast<hipStream_t>(stream), &space);
+}

@@ -14,6 +14,11 @@ namespace pg {
 
 constexpr int kTile = 8;  // tile edge (cells): one cell per lane of a wave
 
+// Bytes of the device array behind pg_propagate_noalloc: the call and launch
+// workspaces of any raster of at most PG_NOALLOC_MAX_CELLS cells (the largest,
+// a 17 x 61680 raster, needs 38.5 MiB on a 256-CU device).
+constexpr size_t kNoAllocScratchBytes = size_t{40} << 20;
+
 struct Geom {
   int W, H;
   float dx, dy, cell, max_time;
@@ -129,7 +134,14 @@ struct Launch {
   int cache;           // all scenarios cache line-of-sight flags
   const uint32_t *lds_bits_src;  // nscen == 1: the (row-major) bitset, staged in LDS if it fits
   size_t lds_bits_words;
+  char *ws;            // launch workspace of launch_ws_bytes(cu count) bytes, or null for the
+                       // per-device one (grown with hipMalloc)
+  int untimed;         // 1: no HIP events (relax_ms stays 0)
 };
+
+// Bytes of a launch workspace (control block, barrier words, profile rows
+// when `profile`) on a device with `cu` compute units.
+size_t launch_ws_bytes(unsigned cu, bool profile);
 
 struct LaunchResult {
   int last_sweep;      // last sweep run

@@ -550,10 +550,12 @@ def moisture_device(fuel, slope, aspect, weather: _native.DayWeather, hour: int,
 
 
 def propagate_device(settings: Settings, speed_max, azimuth_max, eccentricity, time, refs_x,
-                     refs_y, boundaries, tuning=None, stream=None) -> dict:
+                     refs_y, boundaries, tuning=None, stream=None, noalloc: bool = False) -> dict:
     """pg_propagate on resident tensors: time (f32) and refs (16-bit, e.g.
     torch.int16 holding the u16 bits) are in-out, boundaries (32-bit words)
-    out.  Synchronises ``stream``; returns the stats dict."""
+    out.  Synchronises ``stream``; returns the stats dict.  ``noalloc``
+    calls pg_propagate_noalloc instead (no device allocation; rasters of at
+    most 1,048,576 cells)."""
     n = len(settings.geo_ref)
     for t, want in ((speed_max, n), (azimuth_max, n), (eccentricity, n), (time, n), (refs_x, n),
                     (refs_y, n), (boundaries, (n + 31) // 32)):
@@ -564,12 +566,14 @@ def propagate_device(settings: Settings, speed_max, azimuth_max, eccentricity, t
     st = Stats()
     tn = _tuning(tuning)
     sc = settings.to_c()
-    rc = _native.load().pg_propagate(C.byref(sc), _dptr(speed_max), _dptr(azimuth_max),
-                                     _dptr(eccentricity), _dptr(time), _dptr(refs_x),
-                                     _dptr(refs_y), _dptr(boundaries),
-                                     C.byref(tn) if tn is not None else None, C.byref(st),
-                                     _stream_ptr(stream))
-    _raise_for(rc, "pg_propagate")
+    lib = _native.load()
+    name = "pg_propagate_noalloc" if noalloc else "pg_propagate"
+    rc = getattr(lib, name)(C.byref(sc), _dptr(speed_max), _dptr(azimuth_max),
+                            _dptr(eccentricity), _dptr(time), _dptr(refs_x),
+                            _dptr(refs_y), _dptr(boundaries),
+                            C.byref(tn) if tn is not None else None, C.byref(st),
+                            _stream_ptr(stream))
+    _raise_for(rc, name)
     return st.as_dict()
 
 

@@ -18,7 +18,7 @@ EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propa
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
            "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture",
            "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs", "pg_write_times_geotiff",
-           "pg_write_refs_gpkg", "pg_write_tiles_gpkg")
+           "pg_write_refs_gpkg", "pg_write_tiles_gpkg", "pg_propagate_noalloc")
 
 
 class GT(C.Structure):
@@ -123,6 +123,8 @@ def load(require_torch: bool = True):
     lib.pg_propagate.restype = C.c_int
     lib.pg_propagate.argtypes = [C.POINTER(SettingsC)] + [vp] * 7 + [C.POINTER(Tuning),
                                                                      C.POINTER(Stats), vp]
+    lib.pg_propagate_noalloc.restype = C.c_int
+    lib.pg_propagate_noalloc.argtypes = lib.pg_propagate.argtypes
     lib.pg_propagate_host.restype = C.c_int
     lib.pg_propagate_host.argtypes = [C.POINTER(SettingsC)] + [vp] * 15 + [C.POINTER(Tuning),
                                                                            C.POINTER(Stats)]
