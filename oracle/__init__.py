@@ -42,22 +42,50 @@ class OracleStats(C.Structure):
                 ("improvements", C.c_int64), ("dda_steps", C.c_int64)]
 
 
+def _open(path):
+    lb = C.CDLL(path)
+    lb.oracle_burn_f32.restype = C.c_size_t
+    lb.oracle_burn_f64.restype = C.c_size_t
+    lb.oracle_propagate.restype = C.c_int
+    lb.oracle_time_from.restype = C.c_float
+    lb.oracle_time_from.argtypes = [C.POINTER(OracleGrid), C.c_float, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                    C.c_int]
+    lb.oracle_dda.restype = C.c_int
+    lb.oracle_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    return lb
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build(ref=False)
-        _lib = C.CDLL(LIB_PATH)
-        _lib.oracle_burn_f32.restype = C.c_size_t
-        _lib.oracle_burn_f64.restype = C.c_size_t
-        _lib.oracle_propagate.restype = C.c_int
-        _lib.oracle_time_from.restype = C.c_float
-        _lib.oracle_time_from.argtypes = [C.POINTER(OracleGrid), C.c_float, C.c_int, C.c_int,
-                                          C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
-                                          C.c_int]
-        _lib.oracle_dda.restype = C.c_int
-        _lib.oracle_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib = _open(LIB_PATH)
     return _lib
+
+
+def build_mutant(m: int, path: str) -> None:
+    """Compile the oracle with planted mistake ``m`` (propag_oracle.c) to ``path``."""
+    subprocess.run(["make", "-s", "-C", HERE, "mutant", f"M={int(m)}", f"OUT={path}"], check=True)
+
+
+class use_library:
+    """Context manager: the module's functions call the library at ``path``
+    (a mutant build) instead of liboracle.so while it is active."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _open(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
 
 
 def ref_lib():

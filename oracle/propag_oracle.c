@@ -43,6 +43,22 @@
 
 #define NO_REF 0xFFFFu
 
+/* Planted mistakes for the tests that check the known answers can fail
+ * (tests/test_oracle_kat.py builds one library per value; the default build
+ * has none):
+ *   1 the blocked-line fallback is ignored (ct kept despite a boundary hit)
+ *   2 the line-of-sight walk starts at the cell instead of the neighbour
+ *   3 the boundary predicate takes the faster side (> for < on speed)
+ *   4 similar_fires compares azimuths with wrap-around
+ *   5 neighbours visited i outer, j inner
+ *   6 ties keep the last candidate (<= for <)
+ *   7 a neighbour referencing the cell is not skipped
+ *   8 a neighbour burning exactly at max_time spreads (<= max_time)
+ *   9 the post-pass drops arrivals equal to max_time (>= for >) */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------- sincos */
 
 void oracle_sincos(double x, double *s, double *c) {
@@ -219,8 +235,12 @@ int oracle_dda(int fx, int fy, int tx, int ty, int32_t *out_xy, int max_out) {
 
 static inline int similar_fires(float s1, float a1, float e1, float s2,
                                 float a2, float e2) {
+  double da = (double)fabsf(a1 - a2);
+#if ORACLE_MUTANT == 4
+  if (da > 3.14159265358979323846) da = 2 * 3.14159265358979323846 - da;
+#endif
   return (double)fabsf(s1 - s2) < 0.1 &&
-         (double)fabsf(a1 - a2) < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
+         da < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
          (double)fabsf(e1 - e2) < 0.05;
 }
 
@@ -241,7 +261,11 @@ void oracle_boundaries(const oracle_grid *g, const float *speed_max,
           changed |= !similar_fires(speed_max[n], azimuth_max[n],
                                     eccentricity[n], speed_max[c],
                                     azimuth_max[c], eccentricity[c]) &&
+#if ORACLE_MUTANT == 3
+                     speed_max[c] > speed_max[n];
+#else
                      speed_max[c] < speed_max[n];
+#endif
         }
       }
       if (changed) boundaries[c >> 5] |= 1u << (c & 31);
@@ -295,15 +319,27 @@ static int relax_cell(const ctx_t *C, int x, int y, float *out_t,
   uint16_t best_rx = NO_REF, best_ry = NO_REF;
   for (int j = -1; j < 2; j++) {
     for (int i = -1; i < 2; i++) {
-      if (i == 0 && j == 0) continue;
+#if ORACLE_MUTANT == 5
+      { int t_ = i; i = j; j = t_; }
+#endif
       int nx = x + i, ny = y + j;
+#if ORACLE_MUTANT == 5
+      { int t_ = i; i = j; j = t_; }
+#endif
+      if (nx == x && ny == y) continue;
       if (!(nx >= 0 && ny >= 0 && nx < W && ny < H)) continue;
       size_t n = (size_t)nx + (size_t)ny * W;
       float n_t = C->time[n];
       float n_s = C->smax[n];
+#if ORACLE_MUTANT == 8
+      if (!(n_t <= max_time && !(n_s == 0.0f))) continue;
+#else
       if (!(n_t < max_time && !(n_s == 0.0f))) continue;
+#endif
       int rfx = C->rx[n], rfy = C->ry[n];
+#if ORACLE_MUTANT != 7
       if (rfx == x && rfy == y) continue;
+#endif
       float ref_t = C->time[(size_t)rfx + (size_t)rfy * W];
       float nt, ct;
       if (C->formula == 1) {
@@ -317,27 +353,38 @@ static int relax_cell(const ctx_t *C, int x, int y, float *out_t,
         ct = time_from0(ref_t, rfx, rfy, x, y, C->k[n], C->g->dx, C->g->dy,
                         C->north_up);
       }
-      if (ct < nt && ct < best_t) {
+#if ORACLE_MUTANT == 6
+#define ORACLE_LESS(a, b) ((a) <= (b))
+#else
+#define ORACLE_LESS(a, b) ((a) < (b))
+#endif
+      if (ct < nt && ORACLE_LESS(ct, best_t)) {
         int cand_rx = rfx, cand_ry = rfy;
         float cand_t = ct;
         dda_t d;
         int px, py;
+#if ORACLE_MUTANT == 2
+        dda_init(&d, x, y, rfx, rfy);
+#else
         dda_init(&d, nx, ny, rfx, rfy);
+#endif
         while (dda_next(&d, &px, &py)) {
           (*dda_steps)++;
           if (is_boundary(C->bnd, (size_t)px + (size_t)py * W)) {
+#if ORACLE_MUTANT != 1
             cand_rx = nx;
             cand_ry = ny;
             cand_t = nt;
+#endif
             break;
           }
         }
-        if (cand_t < best_t) {
+        if (ORACLE_LESS(cand_t, best_t)) {
           best_t = cand_t;
           best_rx = (uint16_t)cand_rx;
           best_ry = (uint16_t)cand_ry;
         }
-      } else if (nt < best_t) {
+      } else if (ORACLE_LESS(nt, best_t)) {
         best_t = nt;
         best_rx = (uint16_t)nx;
         best_ry = (uint16_t)ny;
@@ -440,7 +487,11 @@ int oracle_propagate(const oracle_grid *g, const float *speed_max,
 
   /* post_propagate */
   for (size_t i = 0; i < N; i++)
+#if ORACLE_MUTANT == 9
+    if (speed_max[i] == 0.0f || time[i] >= g->max_time) time[i] = FLT_MAX;
+#else
     if (speed_max[i] == 0.0f || time[i] > g->max_time) time[i] = FLT_MAX;
+#endif
   free(k);
   if (stats) *stats = st;
   return hit_cap;

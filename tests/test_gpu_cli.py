@@ -1,11 +1,13 @@
 """The `propag` command-line program (the reference's crates/propag/src/main.rs
 on the MI355X path): its fixed README scenario - 1024 x 1024, 5 m, fuel
 model 1, moisture 0.1, wind 5 m/s toward 0, point ignition at (500, 500),
-10 h.  The reference printed num_times_after=447822 and max_time 35999.9
-(README.md:92-113) from its older tiled build; the exact-schedule result is
-450,599 burned cells, the closed-form ellipse count that the oracle is pinned
-to (tests/test_oracle_propag.py) and pg_propagate reproduces
-(tests/test_gpu_propagate.py).  The GeoTIFF it writes must hold the same
+10 h.  The reference's README printed num_times_after=447822 and max_time
+35999.9 (README.md:92-113), from a build older than its current main.rs; the
+result here is 450,599 burned cells, the closed-form ellipse count that the
+oracle is pinned to (tests/test_oracle_propag.py) and pg_propagate reproduces
+(tests/test_gpu_propagate.py).  The 0.6% gap is unexplained: neither the
+literal atan2f/cosf time_from, directed-rounding reciprocal and square root,
+nor the reference's super-tile schedule changes the oracle's count.  The GeoTIFF it writes must hold the same
 burned cells and extreme times as it prints.
 """
 import os

@@ -7,9 +7,11 @@
   reference's atan2/cos formula (propag.cu:269-294), both orientations and
   non-square cells.
 * Uniform fuel: arrival times equal the closed-form ellipse from the
-  ignition; the reference CLI scenario (README.md:92-113) burns the same
-  number of cells as the closed form, within 1% of the reference's printed
-  447,822 (its older tiled build rounds ties differently).
+  ignition; the reference CLI scenario (README.md:92-113) burns exactly the
+  closed form's 450,599 cells, within 1% of the 447,822 the reference's
+  README printed.  That 0.6% gap is unexplained: the literal atan2f/cosf
+  time_from (formula 1), IEEE directed-rounding reciprocal and square root,
+  and the reference's super-tile schedule all give 450,599 here.
 * No wind, flat: the burned set is the disk of radius speed0 * max_time.
 * Boundaries against a vectorised numpy restatement of propag.cu:296-375.
 * Fixed-point invariants, schedule (thread count) independence, edge cases.
