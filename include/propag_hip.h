@@ -52,7 +52,10 @@ typedef struct pg_tuning {
   uint32_t max_iterations;      /* Jacobi sweep cap (0 = 64 * (W + H) + 1024)  */
   uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep,
                                    2 = persistent kernel through a cached hipGraph */
-  uint32_t spin_limit_log2;     /* device wait limit, 2^n polls (0 = 22)       */
+  uint32_t spin_limit_log2;     /* device wait limit, 2^n polls (0 = 22; 32 or
+                                   more = no wait: a grid barrier that finds a
+                                   workgroup not yet arrived fails the launch
+                                   with PG_ERR_TIMEOUT, for tests of that path) */
   uint32_t waves_per_group;     /* persistent kernel waves per workgroup (0 = auto) */
   uint32_t profile;             /* 1: time the sweep phases (fills pg_stats.phase_ms) */
 } pg_tuning;
@@ -286,8 +289,10 @@ typedef struct pg_ignition {
   float time;
 } pg_ignition;
 
-/* Scenario states reported by pg_scenario_status. */
-enum { PG_SCENARIO_RUNNING = 0, PG_SCENARIO_DONE = 1, PG_SCENARIO_NOT_CONVERGED = 2 };
+/* Scenario states reported by pg_scenario_status.  FAILED: a launch that
+ * ran the scenario returned an error (PG_ERR_TIMEOUT, PG_ERR_HIP); it has no
+ * result and is reported by pg_ctx_take_finished like a finished one. */
+enum { PG_SCENARIO_RUNNING = 0, PG_SCENARIO_DONE = 1, PG_SCENARIO_NOT_CONVERGED = 2, PG_SCENARIO_FAILED = 3 };
 
 /* Creates a context on the current device.  hbm_budget caps the bytes the
  * context allocates for scenarios (0: 90 % of the free device memory). */
@@ -316,7 +321,10 @@ pg_status pg_scenario_admit(pg_ctx *ctx, int terrain_id, const pg_weather *weath
  * launch, for at most max_sweeps sweeps (0: until all finish).  Scenarios
  * that finish get their post pass; *n_finished (may be NULL) receives how
  * many finished in this call.  stats (may be NULL) receives the launch's
- * counters. */
+ * counters.  When the launch fails, every scenario it was running becomes
+ * PG_SCENARIO_FAILED (queued in the finished list; release it) and the
+ * error is returned; finished scenarios and later admissions are not
+ * affected. */
 pg_status pg_batch_run(pg_ctx *ctx, uint32_t max_sweeps, const pg_tuning *tuning,
                        pg_stats *stats, uint32_t *n_finished);
 

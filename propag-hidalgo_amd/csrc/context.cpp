@@ -197,6 +197,33 @@ const Scen *dead_descriptor_ptr() {
   return &dead;
 }
 
+// A launch that returns an error (a barrier wait past its spin limit, a HIP
+// failure) has consumed part of a sweep: the tile queues and the state of
+// every running scenario are part-way through it.  Those scenarios become
+// PG_SCENARIO_FAILED (dead descriptors, so no later launch touches their
+// buffers; reported by pg_ctx_take_finished, to be released by the caller),
+// the queues are cleared, and `st` is returned with its message.  Finished
+// scenarios keep their results; later admissions run normally.
+pg_status fail_running(pg_ctx *c, pg_status st) {
+  const std::string msg = pg_last_error();
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipGetLastError();
+  for (size_t i = 0; i < c->scen.size(); ++i) {
+    Scenario &s = c->scen[i];
+    s.zombie = false;  // the cleared queues hold no entry of any slot
+    if (!s.used || s.state != PG_SCENARIO_RUNNING) continue;
+    s.state = PG_SCENARIO_FAILED;
+    (void)put_descriptor(c, static_cast<int>(i), dead_descriptor_ptr());
+    c->finished.push_back(static_cast<int>(i));
+  }
+  if (c->q_block) {
+    (void)scen_fill_u32(reinterpret_cast<uint32_t *>(c->q.list), 4 * c->q.cap * 2, 0xFFFFFFFFu, c->stream);
+    (void)scen_fill_u32(c->q.count, 128, 0u, c->stream);
+  }
+  (void)sync_ctx(c);
+  return set_error(st, msg + " (the running scenarios of the context are marked failed)");
+}
+
 }  // namespace
 }  // namespace pg
 
@@ -467,7 +494,7 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   pg_stats local{};
   LaunchResult res{};
   pg_status st = run_sweeps(c->dev, L, tuning, &local, &res, c->stream);
-  if (st != PG_OK) return st;
+  if (st != PG_OK) return fail_running(c, st);
   std::vector<ScenOut> outs(nslots);
   PG_HIP(hipMemcpyAsync(outs.data(), c->d_out, nslots * sizeof(ScenOut), hipMemcpyDeviceToHost, c->stream));
   PG_HIP(hipStreamSynchronize(c->stream));
@@ -538,6 +565,8 @@ pg_status pg_scenario_result(pg_ctx *c, int id, float *time, uint16_t *refs_x, u
   if (id < 0 || static_cast<size_t>(id) >= c->scen.size() || !c->scen[id].used)
     return set_error(PG_ERR_INVALID, "unknown scenario id");
   const Scenario &s = c->scen[id];
+  if (s.state == PG_SCENARIO_FAILED)
+    return set_error(PG_ERR_INVALID, "scenario failed (its launch returned an error): no result; release it");
   if (s.state == PG_SCENARIO_RUNNING || !s.posted) return set_error(PG_ERR_BUSY, "scenario still running");
   const size_t n = static_cast<size_t>(s.sc.g.W) * s.sc.g.H;
   if (time) PG_HIP(hipMemcpyAsync(time, s.time, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));

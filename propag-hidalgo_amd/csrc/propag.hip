@@ -1877,7 +1877,10 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   a.slots = reinterpret_cast<Slot *>(base + o_slots);
   a.arrive = reinterpret_cast<unsigned long long *>(a.slots + max_groups);
   a.arrive_stride = static_cast<unsigned>(max_groups);
-  a.spin_limit = 1u << (tn.spin_limit_log2 ? (tn.spin_limit_log2 > 30 ? 30 : tn.spin_limit_log2) : 22);
+  // 0: the default 2^22 polls; 32 and above: none (a barrier that finds a
+  // workgroup not yet arrived gives up at once: tests of the failure path)
+  a.spin_limit = tn.spin_limit_log2 >= 32 ? 0u
+                 : 1u << (tn.spin_limit_log2 ? (tn.spin_limit_log2 > 30 ? 30 : tn.spin_limit_log2) : 22);
   a.prof = prof_rows ? reinterpret_cast<unsigned long long *>(base + o_prof) : nullptr;
   a.prof_sweeps = prof_rows;
 

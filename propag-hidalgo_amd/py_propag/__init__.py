@@ -606,7 +606,7 @@ __all__ = [
 
 # ------------------------------------------------------------ batching / serving
 
-RUNNING, DONE, NOT_CONVERGED = 0, 1, 2
+RUNNING, DONE, NOT_CONVERGED, FAILED = 0, 1, 2, 3
 
 
 class Context:
@@ -699,7 +699,9 @@ class Context:
 
     def run(self, max_sweeps: int = 0, tuning=None):
         """One shared launch of all unfinished scenarios (max_sweeps 0: to
-        completion).  Returns (number finished, stats dict)."""
+        completion).  Returns (number finished, stats dict).  If the launch
+        fails, the scenarios it ran become FAILED (listed by
+        ``take_finished``, without results) and the error is raised."""
         stats = Stats()
         fin = C.c_uint32()
         t = _tuning(tuning)

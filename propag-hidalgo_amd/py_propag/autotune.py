@@ -21,6 +21,10 @@ from __future__ import annotations
 import itertools
 import time
 
+import numpy as np
+
+from . import PropagCudaError
+
 
 def default_space():
     """Candidate tunings: waves per workgroup x fraction of the workgroups
@@ -35,13 +39,14 @@ class Autotuner:
     """Times candidate tunings of ``Context.run`` on sample ensembles and
     remembers the best per workload key."""
 
-    def __init__(self, ctx, space=None, repeats: int = 1, clock=time.perf_counter):
+    def __init__(self, ctx, space=None, repeats: int = 2, clock=time.perf_counter):
         self.ctx = ctx
         self.clock = clock
         self.space = space if space is not None else default_space()
         self.repeats = repeats
         self.best = {}     # key -> tuning dict
         self.tables = {}   # key -> [(tuning, seconds)]
+        self.rejected = []  # (tuning, error) of candidates whose launch failed
 
     @staticmethod
     def key(width: int, height: int, count: int):
@@ -52,18 +57,23 @@ class Autotuner:
         return (p2(width), p2(height), p2(count))
 
     def _trial(self, terrain_id, cases, tuning):
-        ids = [self.ctx.admit(terrain_id, c["wx"], c["ign"], c["max_time"]) for c in cases]
-        if any(i is None for i in ids):
+        """Runs the sample once with ``tuning``; the admitted scenarios are
+        released whatever happens.  A launch error propagates (the
+        scenarios are then FAILED and released here)."""
+        ids = []
+        try:
+            for c in cases:
+                i = self.ctx.admit(terrain_id, c["wx"], c["ign"], c["max_time"])
+                if i is None:
+                    raise RuntimeError("autotune sample does not fit the memory budget")
+                ids.append(i)
+            t0 = self.clock()
+            fin, stats = self.ctx.run(tuning=tuning)
+            dt = self.clock() - t0
+            sweeps = [self.ctx.status(i)[1] for i in ids]
+        finally:
             for i in ids:
-                if i is not None:
-                    self.ctx.release(i)
-            raise RuntimeError("autotune sample does not fit the memory budget")
-        t0 = self.clock()
-        fin, stats = self.ctx.run(tuning=tuning)
-        dt = self.clock() - t0
-        sweeps = [self.ctx.status(i)[1] for i in ids]
-        for i in ids:
-            self.ctx.release(i)
+                self.ctx.release(i)
         if fin != len(ids):
             raise RuntimeError("autotune trial left scenarios unfinished")
         return dt, sweeps, stats
@@ -86,14 +96,20 @@ class Autotuner:
                 continue  # needs the full count of this shape first
             groups = 0 if cand["group_fraction"] >= 1.0 else max(1, int(full[w] * cand["group_fraction"]))
             tuning = {"waves_per_group": w, "mode": cand["mode"], "workgroups": groups}
-            best_dt = None
-            for _ in range(self.repeats):
-                dt, sweeps, st = self._trial(terrain_id, cases, tuning)
-                if sweeps != ref_sweeps:
-                    raise RuntimeError(f"tuning {tuning} changed the sweep counts")
-                best_dt = dt if best_dt is None else min(best_dt, dt)
+            times = []
+            try:
+                for _ in range(self.repeats):
+                    dt, sweeps, st = self._trial(terrain_id, cases, tuning)
+                    if sweeps != ref_sweeps:
+                        raise RuntimeError(f"tuning {tuning} changed the sweep counts")
+                    times.append(dt)
+            except PropagCudaError as e:
+                # a launch error (e.g. a barrier past its spin limit) rejects
+                # the candidate; its sample was released
+                self.rejected.append((tuning, str(e)))
+                continue
             full.setdefault(w, st["workgroups"])
-            table.append(({**tuning, "workgroups": st["workgroups"]}, best_dt))
+            table.append(({**tuning, "workgroups": st["workgroups"]}, float(np.median(times))))
         table.sort(key=lambda e: e[1])
         self.tables[k] = table
         self.best[k] = table[0][0]

@@ -22,7 +22,7 @@ import collections
 import time
 from dataclasses import dataclass, field
 
-from . import Context
+from . import FAILED, Context, PropagCudaError
 
 
 @dataclass
@@ -63,6 +63,8 @@ class Server:
         self.kernel_ms = 0.0         # relaxation kernel time (HIP events)
         self.sweeps_run = 0
         self.busy_refusals = 0
+        self.failed_launches = 0
+        self.last_error = None
 
     def submit(self, req: Request):
         self.queue.append(req)
@@ -85,27 +87,35 @@ class Server:
 
     def step(self):
         """Admits what fits, runs one launch, delivers what finished.
-        Returns the requests finished in this step."""
+        Returns the requests finished in this step.  A launch that fails
+        fails the requests it was running (state FAILED, ``on_done`` not
+        called); the loop goes on with the others."""
         self._admit(self.clock())
         if not self.running:
             return []
         t0 = self.clock()
-        _, stats = self.ctx.run(max_sweeps=self.sweeps, tuning=self.tuning)
+        try:
+            _, stats = self.ctx.run(max_sweeps=self.sweeps, tuning=self.tuning)
+        except PropagCudaError as e:
+            self.failed_launches += 1
+            self.last_error = str(e)
+            stats = None
         ms = (self.clock() - t0) * 1e3
         self.launches += 1
         self.launch_ms += ms
-        self.kernel_ms += stats["relax_ms"]
-        self.sweeps_run += stats["iterations"]
-        if ms > 0:
-            want = int(self.sweeps * self.target_ms / ms)
-            self.sweeps = max(self.min_sweeps, min(self.max_sweeps, (self.sweeps + want) // 2))
+        if stats is not None:
+            self.kernel_ms += stats["relax_ms"]
+            self.sweeps_run += stats["iterations"]
+            if ms > 0:
+                want = int(self.sweeps * self.target_ms / ms)
+                self.sweeps = max(self.min_sweeps, min(self.max_sweeps, (self.sweeps + want) // 2))
         done = []
         for sid in self.ctx.take_finished():
             req = self.running.pop(sid, None)
             if req is None:
                 continue
             req.state, req.sweeps = self.ctx.status(sid)
-            if self.on_done is not None:
+            if self.on_done is not None and req.state != FAILED:
                 self.on_done(req, sid)
             req.finished = self.clock()
             self.ctx.release(sid)

@@ -323,3 +323,55 @@ def test_autotuner_picks_a_tuning_without_changing_results():
         ctx.run(tuning=best)
         for sid, c in zip(ids, cases):
             _same(ctx.result(sid), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+
+
+def test_failed_launch_fails_its_running_scenarios_only():
+    """A launch that returns an error (here the grid barrier told to give up
+    at its first wait: spin_limit_log2 = 32, so every kernel still exits
+    normally) fails the scenarios it was running: they report FAILED through
+    take_finished, have no result and are released; a scenario finished
+    before keeps its result, and later admissions run normally.  The plain
+    entry point recovers the same way on its next call."""
+    import py_propag as pp
+    from py_propag.serve import Request, Server
+
+    w, h = 160, 120
+    land = _terrain(12, w, h)
+    a, b, c, d = (_case(s, w, h) for s in (110, 111, 112, 113))
+    stop = {"spin_limit_log2": 32}
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        ia = ctx.admit(tid, a["wx"], a["ign"], a["max_time"])
+        ctx.run()
+        assert ctx.status(ia)[0] == pp.DONE
+        assert ctx.take_finished() == [ia]
+        ib = ctx.admit(tid, b["wx"], b["ign"], b["max_time"])
+        ic = ctx.admit(tid, c["wx"], c["ign"], c["max_time"])
+        with pytest.raises(pp.PropagCudaError, match="PG_ERR_TIMEOUT"):
+            ctx.run(tuning=stop)
+        assert ctx.status(ib)[0] == pp.FAILED and ctx.status(ic)[0] == pp.FAILED
+        assert sorted(ctx.take_finished()) == sorted([ib, ic])
+        with pytest.raises(pp.PropagValidationError, match="failed"):
+            ctx.result(ib)
+        ctx.release(ib)
+        ctx.release(ic)
+        _same(ctx.result(ia), _alone(pp, land, w, h, a["wx"], a["ign"], a["max_time"]))
+        idd = ctx.admit(tid, d["wx"], d["ign"], d["max_time"])
+        ctx.run()
+        assert ctx.status(idd)[0] == pp.DONE
+        _same(ctx.result(idd), _alone(pp, land, w, h, d["wx"], d["ign"], d["max_time"]))
+        ctx.release(idd)
+        ctx.release(ia)
+        # the request loop delivers failed requests as FAILED, without on_done
+        got = []
+        server = Server(ctx, on_done=lambda req, sid: got.append(req.tag), tuning=stop)
+        done = server.play([Request(tid, b["wx"], b["ign"], b["max_time"], 0.0, tag="b")])
+        assert [r.state for r in done] == [pp.FAILED] and got == [] and server.failed_launches == 1
+        assert ctx.take_finished() == []
+    # the one-scenario entry point: the failing call raises, the next one is exact
+    settings = pp.Settings(_geo(pp, w, h), b["max_time"])
+    with pytest.raises(pp.PropagCudaError, match="PG_ERR_TIMEOUT"):
+        pp.propagate(settings, _full_terrain(land, b["wx"], w * h), ignitions=b["ign"], tuning=stop)
+    again = pp.propagate(settings, _full_terrain(land, b["wx"], w * h), ignitions=b["ign"])
+    ref = _alone(pp, land, w, h, b["wx"], b["ign"], b["max_time"])
+    assert np.array_equal(again.arrival_times.view(np.uint32), ref.arrival_times.view(np.uint32))
