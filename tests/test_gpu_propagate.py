@@ -200,3 +200,96 @@ def test_large_scenario_4096_fixed_point(oracle_lib):
     again = gpu_propagate(sc, fire, time=first["time"], refs=(first["refs_x"], first["refs_y"]))
     assert np.array_equal(again["time"].view(np.uint32), first["time"].view(np.uint32))
     print("4096^2 stats:", first["stats"])
+
+
+
+def _wide_scenario(W, H, seed):
+    """A W x H band (sides past 32767 select the kernels with full 16-bit
+    reference coordinates and line-of-sight walks at evaluation time):
+    fuel codes in 64 x 8 blocks, one moisture draw, wind 6 m/s toward
+    80 deg, slope 0.05 with blockwise aspect, 2 h, two ignitions near the east
+    end (past x = 32767 when the band is wider than that)."""
+    r = np.random.Generator(np.random.PCG64(seed))
+    bx, by = (W + 63) // 64, (H + 7) // 8
+    blocks = lambda v: np.kron(v, np.ones((8, 64), v.dtype))[:H, :W].ravel()  # noqa: E731
+    n = W * H
+    d1 = np.float32(r.uniform(0.03, 0.12))
+    terrain = {"fuel": np.ascontiguousarray(blocks(r.integers(1, 14, (by, bx)).astype(np.uint8))),
+               "d1hr": np.full(n, d1, np.float32), "d10hr": np.full(n, d1 + np.float32(0.01), np.float32),
+               "d100hr": np.full(n, d1 + np.float32(0.02), np.float32),
+               "herb": np.full(n, np.float32(r.uniform(0.3, 1.5)), np.float32),
+               "wood": np.full(n, np.float32(r.uniform(0.6, 1.5)), np.float32),
+               "wind_speed": np.full(n, 6.0, np.float32),
+               "wind_azimuth": np.full(n, np.float32(np.radians(80.0)), np.float32),
+               "slope": np.full(n, 0.05, np.float32),
+               "aspect": np.ascontiguousarray(blocks(r.uniform(0, 2 * np.pi, (by, bx)).astype(np.float32)))}
+    ign = [(W - 6500, H // 2, 0.0), (W - 3800, 3, 300.0)]
+    time, rx, ry = synth.initial_state(W, H, ign)
+    return {"terrain": terrain, "width": W, "height": H, "dx": 5.0, "dy": 5.0, "max_time": np.float32(7200.0),
+            "time": time, "refs_x": rx, "refs_y": ry, "ignitions": ign}
+
+
+def _raw_settings(sc):
+    """Settings through the C structs directly: the Python GeoReference
+    keeps the reference's 32768 limit, the C ABI takes sides up to 65535."""
+    from py_propag import _native
+
+    s = _native.SettingsC()
+    s.geo_ref.width, s.geo_ref.height = sc["width"], sc["height"]
+    s.geo_ref.proj[:len(b"+proj=utm +zone=30")] = b"+proj=utm +zone=30"
+    gt = s.geo_ref.transform.gt
+    gt.x0, gt.dx, gt.rx, gt.y0, gt.ry, gt.dy = 0.0, sc["dx"], 0.0, 0.0, 0.0, sc["dy"]
+    inv = s.geo_ref.transform.inv
+    inv.x0, inv.dx, inv.rx, inv.y0, inv.ry, inv.dy = 0.0, 1.0 / sc["dx"], 0.0, 0.0, 0.0, 1.0 / sc["dy"]
+    s.max_time = float(sc["max_time"])
+    return s
+
+
+def _gpu_propagate_raw(sc, fire, tuning):
+    import ctypes as C
+    import torch
+    import py_propag as pp
+    from py_propag import _native
+
+    n = sc["width"] * sc["height"]
+    dev = torch.device("cuda")
+    f = [torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev) for x in fire]
+    t = torch.from_numpy(sc["time"].copy()).to(dev)
+    rx = torch.from_numpy(sc["refs_x"].view(np.int16).copy()).to(dev)
+    ry = torch.from_numpy(sc["refs_y"].view(np.int16).copy()).to(dev)
+    b = torch.zeros((n + 31) // 32, dtype=torch.int32, device=dev)
+    tn = _native.Tuning()
+    for k, v in tuning.items():
+        setattr(tn, k, v)
+    st = _native.Stats()
+    s = _raw_settings(sc)
+    ptr = lambda x: C.c_void_p(x.data_ptr())  # noqa: E731
+    rc = _native.load().pg_propagate(C.byref(s), *[ptr(x) for x in f], ptr(t), ptr(rx), ptr(ry), ptr(b),
+                                     C.byref(tn), C.byref(st), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0, _native.last_error()
+    return {"time": t.cpu().numpy(), "refs_x": rx.cpu().numpy().view(np.uint16),
+            "refs_y": ry.cpu().numpy().view(np.uint16), "boundaries": b.cpu().numpy().view(np.uint32),
+            "stats": st.as_dict()}
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("W,H,lds", [(32768, 16, 1), (32768, 48, 0), (40000, 16, 1)])
+def test_wide_rasters_uncached_path(oracle_lib, W, H, lds, mode):
+    """Bands 32768 and 40000 cells wide, where references need all 16 bits
+    of a coordinate: the boundary bitset in LDS (H = 16) and in L2
+    (H = 48), persistent and per-sweep launches, bit-exact vs the oracle.
+    The 40000-wide band goes through the C ABI directly (the Python
+    GeoReference keeps the reference's 32768 limit)."""
+    sc = _wide_scenario(W, H, seed=H)
+    fire = _fire(sc)
+    ref = oracle_propagate(sc, fire)
+    if W > 32768:
+        gpu = _gpu_propagate_raw(sc, fire, {"mode": mode})
+        assert ref["refs_x"][ref["time"] < FLT_MAX].max() > 32767
+    else:
+        gpu = gpu_propagate(sc, fire, tuning={"mode": mode})
+    assert (ref["time"] < FLT_MAX).sum() > 5000
+    assert_parity(gpu, ref, 7200.0)
+    assert gpu["stats"]["iterations"] == ref["sweeps"]
+    if mode == 0:
+        assert gpu["stats"]["lds_bitset"] == lds
