@@ -169,7 +169,7 @@ pg_status pg_propagate(const Settings *settings, const float *speed_max,
  * that forbid hipMalloc).  It has no counterpart in the reference.  Same
  * arguments, results and errors as pg_propagate, but every buffer beyond the
  * caller's inputs and outputs comes from a fixed device array of the library
- * (40 MiB, held on each device the library runs on until the process ends),
+ * (44 MiB, held on each device the library runs on until the process ends),
  * and the call makes no hipMalloc/hipFree, creates no events and caches no
  * graph.  Limits: at most PG_NOALLOC_MAX_CELLS cells (larger rasters return
  * PG_ERR_INVALID), tuning->mode 0 or 1 (2 and profile = 1 return

@@ -118,30 +118,39 @@ size_t live_tiles(const pg_ctx *c) {
 pg_status grow_queue(pg_ctx *c, size_t cap) {
   if (cap <= c->q.cap) return PG_OK;
   cap = std::max(cap, 2 * c->q.cap);
+  // Eight sub-lists per list, one per XCD (each able to hold every tile).
+  const size_t nsub = kXcds;
   const size_t list_bytes = align_up(cap * sizeof(unsigned long long), 256);
+  const size_t count_words = 4 * nsub * 32;
   void *blk = nullptr;
-  hipError_t e = hipMalloc(&blk, 4 * list_bytes + 512);
+  hipError_t e = hipMalloc(&blk, 4 * nsub * list_bytes + count_words * 4);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return set_error(PG_ERR_OUT_OF_MEMORY, "tile queue allocation failed");
   }
   TileQueue nq{};
   nq.list = reinterpret_cast<unsigned long long *>(blk);
-  nq.count = reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + 4 * list_bytes);
+  nq.count = reinterpret_cast<uint32_t *>(static_cast<char *>(blk) + 4 * nsub * list_bytes);
   nq.cap = list_bytes / sizeof(unsigned long long);
-  pg_status st = scen_fill_u32(reinterpret_cast<uint32_t *>(blk), 4 * list_bytes / 4, 0xFFFFFFFFu, c->stream);
+  nq.nsub = static_cast<uint32_t>(nsub);
+  pg_status st = scen_fill_u32(reinterpret_cast<uint32_t *>(blk), 4 * nsub * list_bytes / 4, 0xFFFFFFFFu, c->stream);
   if (st != PG_OK) return st;
-  st = scen_fill_u32(nq.count, 128, 0u, c->stream);
+  st = scen_fill_u32(nq.count, count_words, 0u, c->stream);
   if (st != PG_OK) return st;
   if (c->q_block) {
-    PG_HIP(hipMemcpyAsync(nq.count, c->q.count, 512, hipMemcpyDeviceToDevice, c->stream));
-    const int p = c->k_next & 3;
-    uint32_t pending = 0;
-    PG_HIP(hipMemcpyAsync(&pending, c->q.count + 32 * p, sizeof(pending), hipMemcpyDeviceToHost, c->stream));
+    // Only the list of sweep k_next holds entries between launches.
+    PG_HIP(hipMemcpyAsync(nq.count, c->q.count, count_words * 4, hipMemcpyDeviceToDevice, c->stream));
+    const size_t p = static_cast<size_t>(c->k_next & 3);
+    uint32_t pending[kXcds] = {};
+    for (size_t x = 0; x < nsub; ++x)
+      PG_HIP(hipMemcpyAsync(&pending[x], c->q.count + 32 * (p * nsub + x), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            c->stream));
     PG_HIP(hipStreamSynchronize(c->stream));
-    if (pending)
-      PG_HIP(hipMemcpyAsync(nq.list + p * nq.cap, c->q.list + p * c->q.cap, pending * sizeof(unsigned long long),
-                            hipMemcpyDeviceToDevice, c->stream));
+    for (size_t x = 0; x < nsub; ++x)
+      if (pending[x])
+        PG_HIP(hipMemcpyAsync(nq.list + (p * nsub + x) * nq.cap, c->q.list + (p * nsub + x) * c->q.cap,
+                              std::min<size_t>(pending[x], c->q.cap) * sizeof(unsigned long long),
+                              hipMemcpyDeviceToDevice, c->stream));
     PG_HIP(hipStreamSynchronize(c->stream));
     PG_HIP(hipFree(c->q_block));
   }
@@ -217,8 +226,9 @@ pg_status fail_running(pg_ctx *c, pg_status st) {
     c->finished.push_back(static_cast<int>(i));
   }
   if (c->q_block) {
-    (void)scen_fill_u32(reinterpret_cast<uint32_t *>(c->q.list), 4 * c->q.cap * 2, 0xFFFFFFFFu, c->stream);
-    (void)scen_fill_u32(c->q.count, 128, 0u, c->stream);
+    (void)scen_fill_u32(reinterpret_cast<uint32_t *>(c->q.list), 4 * c->q.nsub * c->q.cap * 2, 0xFFFFFFFFu,
+                        c->stream);
+    (void)scen_fill_u32(c->q.count, 4 * c->q.nsub * 32, 0u, c->stream);
   }
   (void)sync_ctx(c);
   return set_error(st, msg + " (the running scenarios of the context are marked failed)");

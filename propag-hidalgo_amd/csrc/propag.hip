@@ -134,6 +134,21 @@ __device__ inline uint2 ld_state(const uint2 *p) {
   const unsigned long long v = ld_u64(reinterpret_cast<const unsigned long long *>(p));
   return make_uint2(static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32));
 }
+// Cacheable forms (workgroup-scope relaxed atomics: plain loads that may hit
+// the CU's L1 and the XCD's L2) for launches that run an agent-scope acquire
+// after every grid barrier, which drops the lines a previous sweep left.
+__device__ inline uint2 ld_state_c(const uint2 *p) {
+  const unsigned long long v = __hip_atomic_load(
+      G(reinterpret_cast<unsigned long long *>(const_cast<uint2 *>(p))), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_WORKGROUP);
+  return make_uint2(static_cast<uint32_t>(v), static_cast<uint32_t>(v >> 32));
+}
+__device__ inline int ld_i32_c(const int *p) {
+  return __hip_atomic_load(G(const_cast<int *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline uint32_t ld_u32_c(const uint32_t *p) {
+  return __hip_atomic_load(G(const_cast<uint32_t *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ inline void st_state(uint2 *p, uint2 v) {
   st_u64(reinterpret_cast<unsigned long long *>(p),
          static_cast<unsigned long long>(v.x) | (static_cast<unsigned long long>(v.y) << 32));
@@ -441,6 +456,7 @@ struct Args {
   unsigned arrive_stride;
   unsigned spin_limit;
   int lds_bits;              // bitset of scenario 0 staged in LDS
+  int cached_stage;          // stage halos with cacheable loads (acquire after each barrier)
   const uint32_t *lds_src;   // that bitset (row-major)
   unsigned lds_words;
   unsigned long long *prof;  // [sweep - k_begin][kProfCols] profile ticks, or null
@@ -549,11 +565,48 @@ constexpr unsigned long long kNoEntry = ~0ull;
 
 // (base + stride, not an array of pointers: a dynamic index into the
 // kernel argument would move the whole argument block to scratch)
-__device__ inline unsigned long long *list_of(const TileQueue &q, int k) {
-  return q.list + static_cast<size_t>(k & 3) * q.cap;
+__device__ inline unsigned long long *list_of(const TileQueue &q, int k, uint32_t x) {
+  return q.list + (static_cast<size_t>(k & 3) * q.nsub + x) * q.cap;
 }
-__device__ inline uint32_t *count_of(const TileQueue &q, int k) { return q.count + 32 * (k & 3); }
+__device__ inline uint32_t *count_of(const TileQueue &q, int k, uint32_t x) {
+  return q.count + 32 * (static_cast<uint32_t>(k & 3) * q.nsub + x);
+}
 __device__ inline uint32_t *mark_of(const Scen &sc, int k) { return (k & 1) ? sc.mark[1] : sc.mark[0]; }
+
+// A list entry is scenario << 32 | xcd << 29 | tile: xcd (0..7) names the
+// XCD whose workgroups run the tile in launches that split work by XCD,
+// the same for all tiles of one 8 x 8-tile block of a scenario, so a tile's
+// halo and its neighbours' stay in one L2 while a fire front spans several
+// XCDs.
+constexpr uint32_t kTileMask = 0x1FFFFFFFu;
+__device__ inline uint32_t entry_low(uint32_t si, int X, int Y, uint32_t t) {
+  const uint32_t tag = (static_cast<uint32_t>(X >> 3) + static_cast<uint32_t>(Y >> 3) * 3u + si * 5u) & 7u;
+  return t | (tag << 29);
+}
+// The sub-list an entry goes to.
+__device__ inline uint32_t sub_of(const TileQueue &q, uint32_t low) { return q.nsub > 1 ? (low >> 29) & 7u : 0u; }
+
+// Appends the entries of the lanes with `fresh` to the sub-lists of sweep
+// k1 that their tags name: the first lane of each sub-list's group adds the
+// group's size to that sub-list's count (all groups in one round trip),
+// then each lane stores its entry in its slot.  Called by all lanes.
+__device__ inline void append_entries(const TileQueue &q, int k1, bool fresh, unsigned long long ev, int lane) {
+  if (__ballot(fresh) == 0ull) return;
+  const uint32_t x = sub_of(q, static_cast<uint32_t>(ev));
+  unsigned long long mine = 0;
+  for (uint32_t t = 0; t < q.nsub; ++t) {
+    const unsigned long long g = __ballot(fresh && x == t);
+    if (x == t) mine = g;
+  }
+  const int leader = mine ? __builtin_ctzll(mine) : lane;
+  uint32_t base = 0;
+  if (fresh && lane == leader) base = add_relaxed(count_of(q, k1, x), static_cast<unsigned>(__builtin_popcountll(mine)));
+  base = __shfl(base, leader);
+  if (fresh) {
+    const size_t slot = base + static_cast<uint32_t>(__builtin_popcountll(mine & ((1ull << lane) - 1ull)));
+    if (slot < q.cap) st_u64(&list_of(q, k1, x)[slot], ev);
+  }
+}
 
 // Queuing a tile for sweep k1 is split in two so that the first round trip
 // can overlap other work: mark_issue sets the tile's bit (lanes with
@@ -562,12 +615,14 @@ __device__ inline uint32_t *mark_of(const Scen &sc, int k) { return (k & 1) ? sc
 struct Mark {
   uint32_t t, bit, old;
   bool valid;
+  uint32_t low;  // the list entry's low word (tile and XCD tag)
 };
 
-__device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, int k1, bool valid, int X, int Y) {
-  Mark m{0u, 0u, 0u, valid};
+__device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k1, bool valid, int X, int Y) {
+  Mark m{0u, 0u, 0u, valid, 0u};
   if (valid) {
     m.t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
+    m.low = entry_low(si, X, Y, m.t);
     m.bit = 1u << (m.t >> sc.nwords_log2);
     m.old = or_fetch(&mark_of(sc, k1)[m.t & ((1u << sc.nwords_log2) - 1u)], m.bit);
   }
@@ -576,16 +631,7 @@ __device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, int k1, bo
 
 __device__ inline void mark_commit(const TileQueue &q, uint32_t si, int k1, const Mark &mk, int lane) {
   const bool fresh = mk.valid && (mk.old & mk.bit) == 0u;
-  const unsigned long long m = __ballot(fresh);
-  if (!m) return;
-  const int leader = __builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = add_relaxed(count_of(q, k1), static_cast<unsigned>(__builtin_popcountll(m)));
-  base = __shfl(base, leader);
-  if (fresh) {
-    const size_t slot = base + static_cast<uint32_t>(__builtin_popcountll(m & ((1ull << lane) - 1ull)));
-    if (slot < q.cap) st_u64(&list_of(q, k1)[slot], (static_cast<unsigned long long>(si) << 32) | mk.t);
-  }
+  append_entries(q, k1, fresh, (static_cast<unsigned long long>(si) << 32) | mk.low, lane);
 }
 
 // Appends the wave's held entries to the list of sweep k1: one atomic add
@@ -593,12 +639,10 @@ __device__ inline void mark_commit(const TileQueue &q, uint32_t si, int k1, cons
 __device__ inline void flush_entries(const TileQueue &q, WalkQueue &wq, int k1, int lane) {
   if (wq.ne == 0) return;
   wave_sync();
-  uint32_t base = 0;
-  if (lane == 0) base = add_relaxed(count_of(q, k1), wq.ne);
-  base = __shfl(base, 0);
-  for (unsigned i = static_cast<unsigned>(lane); i < wq.ne; i += kWave) {
-    const size_t slot = static_cast<size_t>(base) + i;
-    if (slot < q.cap) st_u64(&list_of(q, k1)[slot], wq.e[i]);
+  for (unsigned i0 = 0; i0 < wq.ne; i0 += kWave) {
+    const unsigned i = i0 + static_cast<unsigned>(lane);
+    const bool has = i < wq.ne;
+    append_entries(q, k1, has, has ? wq.e[i] : 0ull, lane);
   }
   wq.ne = 0;
   wave_sync();
@@ -614,7 +658,7 @@ __device__ inline void mark_hold(const TileQueue &q, WalkQueue &wq, uint32_t si,
   if (wq.ne + nf > static_cast<unsigned>(kEntryQ)) flush_entries(q, wq, k1, lane);
   if (fresh)
     wq.e[wq.ne + static_cast<unsigned>(__builtin_popcountll(m & ((1ull << lane) - 1ull)))] =
-        (static_cast<unsigned long long>(si) << 32) | mk.t;
+        (static_cast<unsigned long long>(si) << 32) | mk.low;
   wq.ne += nf;
 }
 
@@ -624,8 +668,10 @@ __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t s
   const uint32_t bit = 1u << (t >> sc.nwords_log2);
   const uint32_t old = atomicOr(&mark_of(sc, k1)[t & ((1u << sc.nwords_log2) - 1u)], bit);
   if (old & bit) return;
-  const size_t slot = atomicAdd(count_of(q, k1), 1u);
-  if (slot < q.cap) list_of(q, k1)[slot] = (static_cast<unsigned long long>(si) << 32) | t;
+  const uint32_t low = entry_low(si, X, Y, t);
+  const uint32_t x = sub_of(q, low);
+  const size_t slot = atomicAdd(count_of(q, k1, x), 1u);
+  if (slot < q.cap) list_of(q, k1, x)[slot] = (static_cast<unsigned long long>(si) << 32) | low;
 }
 
 // The tiles a change of tile (tx,ty) in sweep k queues for sweep k + 1: its
@@ -656,7 +702,7 @@ __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
 // lie within two cells of r.  So a neighbour tile is queued only if a
 // changed cell lies within two cells of the shared edge (or corner); the
 // tile itself is queued whenever it changed.
-__device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, int k, int tx, int ty,
+__device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k, int tx, int ty,
                                          const NextTiles &nt, unsigned long long imp, int lane) {
   if (lane < 9) {
     const int dx = lane % 3 - 1, dy = lane / 3 - 1;
@@ -664,11 +710,11 @@ __device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, int
     const unsigned long long selx = dx < 0 ? kCol01 : (dx > 0 ? kCol01 << (kTile - 2) : ~0ull);
     const unsigned long long sely = dy < 0 ? kRow01 : (dy > 0 ? kRow01 << (kTile * (kTile - 2)) : ~0ull);
     const int X = tx + dx, Y = ty + dy;
-    return mark_issue(q, sc, k + 1, (imp & selx & sely) != 0ull && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty,
+    return mark_issue(q, sc, si, k + 1, (imp & selx & sely) != 0ull && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty,
                       X, Y);
   }
   const int b = lane - 9;
-  return mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+  return mark_issue(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
 }
 
 // Rounds 1.. of the box; returns the number of tiles marked in all rounds.
@@ -677,7 +723,7 @@ __device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_
                                      WalkQueue &wq, int lane) {
   for (int b0 = kWave - 9; b0 < nt.n; b0 += kWave) {
     const int b = b0 + lane;
-    const Mark mk = mark_issue(q, sc, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+    const Mark mk = mark_issue(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
     if (kHold)
       mark_hold(q, wq, si, k + 1, mk, lane);
     else
@@ -823,8 +869,13 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const int gx = ox + h % kHalo - 1, gy = oy + h / kHalo - 1;
     const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
     const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
-    w[it] = inb ? ld_state(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
-    stp[it] = inb ? ld_i32(&stamp[gi]) : kStampNever;
+    if (a.cached_stage) {
+      w[it] = inb ? ld_state_c(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
+      stp[it] = inb ? ld_i32_c(&stamp[gi]) : kStampNever;
+    } else {
+      w[it] = inb ? ld_state(&st_in[gi]) : make_uint2(__float_as_uint(FLT_MAX), kNoRef);
+      stp[it] = inb ? ld_i32(&stamp[gi]) : kStampNever;
+    }
     cf[it] = coef[gi];
   }
   float rt[kPer];
@@ -837,8 +888,13 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const bool gather = t < g.max_time && t < FLT_MAX;
     const size_t ri =
         gather ? static_cast<size_t>(ref_x<kCache>(w[it].y)) + static_cast<size_t>(ref_y<kCache>(w[it].y)) * g.W : 0;
-    rt[it] = gather ? __uint_as_float(ld_u32(&st_in[ri].x)) : FLT_MAX;
-    rstp[it] = gather ? ld_i32(&stamp[ri]) : kStampNever;
+    if (a.cached_stage) {
+      rt[it] = gather ? __uint_as_float(ld_u32_c(&st_in[ri].x)) : FLT_MAX;
+      rstp[it] = gather ? ld_i32_c(&stamp[ri]) : kStampNever;
+    } else {
+      rt[it] = gather ? __uint_as_float(ld_u32(&st_in[ri].x)) : FLT_MAX;
+      rstp[it] = gather ? ld_i32(&stamp[ri]) : kStampNever;
+    }
   }
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
@@ -940,7 +996,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const unsigned long long refs =
         static_cast<unsigned long long>(__shfl(bx, 4)) | (static_cast<unsigned long long>(__shfl(bx, 5)) << 32);
     if (imp & refs) nt = next_tiles(sc, bx);
-    mk0 = next_round0_issue(a.q, sc, k, tx, ty, nt, imp, lane);
+    mk0 = next_round0_issue(a.q, sc, si, k, tx, ty, nt, imp, lane);
   }
   if (need) {
     const int me = lx + ly * kHalo;
@@ -1051,7 +1107,7 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
                                  WalkQueue &wq, Counters &cnt) {
   const unsigned lane = threadIdx.x & (kWave - 1);
   const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
-  const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev));
+  const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev) & kTileMask);
   if (si >= static_cast<uint32_t>(a.nscen)) return false;
   const Scen &sc = a.scen[si];
   if (sc.max_iters == 0u) return false;  // released scenario: its buffers are gone
@@ -1080,24 +1136,34 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
                                Counters &cnt) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
-  unsigned long long *list = list_of(a.q, k);
-  auto draw = [&](size_t &e) {
-    unsigned i = 0;
-    if (lane == 0) i = atomicAdd(next, 1u);
-    i = __shfl(i, 0);
-    e = blockIdx.x + static_cast<size_t>(gridDim.x) * i;
-    return e < a.q.cap ? ld_u64(&list[e]) : kNoEntry;
-  };
+  // Workgroups fall in nx classes, w in class w % nx; class c takes the
+  // sub-lists c, c + nx, ... and its workgroups share them, workgroup w
+  // taking entries w / nx, w / nx + G / nx, ... of each.  With one sub-list
+  // per XCD and G a multiple of 8, nx = 8: workgroup w (on XCD w % 8) runs
+  // sub-list w % 8.  next[x] counts the draws from sub-list x.
+  const unsigned G = gridDim.x;
+  const unsigned nx = a.q.nsub > 1 ? (G < kXcds ? G : kXcds) : 1u;
+  const unsigned r = blockIdx.x / nx, nr = G / nx;
   bool changed = false;
-  size_t e;
-  unsigned long long v = draw(e);
-  while (v != kNoEntry) {
-    if (lane == 0) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
-    size_t e2;
-    const unsigned long long v2 = draw(e2);  // in flight while this tile runs
-    changed |= run_entry<kCache, kLdsBits>(a, v, lds_off, k, s, wq, cnt);
-    v = v2;
-    e = e2;
+  for (uint32_t x = blockIdx.x % nx; x < a.q.nsub; x += nx) {
+    unsigned long long *list = list_of(a.q, k, x);
+    auto draw = [&](size_t &e) {
+      unsigned i = 0;
+      if (lane == 0) i = atomicAdd(&next[x], 1u);
+      i = __shfl(i, 0);
+      e = r + static_cast<size_t>(nr) * i;
+      return e < a.q.cap ? ld_u64(&list[e]) : kNoEntry;
+    };
+    size_t e;
+    unsigned long long v = draw(e);
+    while (v != kNoEntry) {
+      if (lane == 0) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
+      size_t e2;
+      const unsigned long long v2 = draw(e2);  // in flight while this tile runs
+      changed |= run_entry<kCache, kLdsBits>(a, v, lds_off, k, s, wq, cnt);
+      v = v2;
+      e = e2;
+    }
   }
   if (defer_walks<kCache, kLdsBits>()) {
     const uint64_t tw = a.prof ? wall_clock64() : 0;
@@ -1119,20 +1185,22 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
                            DeferLds *dq, Counters &cnt) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
-  unsigned long long *list = list_of(a.q, k);
   bool changed = false;
-  for (size_t e0 = gwave; e0 < a.q.cap; e0 += static_cast<size_t>(nwaves) * kWave) {
-    const size_t e = e0 + static_cast<size_t>(lane) * nwaves;
-    unsigned long long v = kNoEntry;
-    if (e < a.q.cap) v = ld_u64(&list[e]);
-    if (v != kNoEntry) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
-    const unsigned long long m = __ballot(v != kNoEntry);
-    for (unsigned long long r = m; r;) {
-      const int l = __builtin_ctzll(r);
-      r &= r - 1;
-      changed |= run_entry<kCache, kLdsBits>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
+  for (uint32_t x = 0; x < a.q.nsub; ++x) {
+    unsigned long long *list = list_of(a.q, k, x);
+    for (size_t e0 = gwave; e0 < a.q.cap; e0 += static_cast<size_t>(nwaves) * kWave) {
+      const size_t e = e0 + static_cast<size_t>(lane) * nwaves;
+      unsigned long long v = kNoEntry;
+      if (e < a.q.cap) v = ld_u64(&list[e]);
+      if (v != kNoEntry) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
+      const unsigned long long m = __ballot(v != kNoEntry);
+      for (unsigned long long r = m; r;) {
+        const int l = __builtin_ctzll(r);
+        r &= r - 1;
+        changed |= run_entry<kCache, kLdsBits>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
+      }
+      if (m != ~0ull) break;
     }
-    if (m != ~0ull) break;
   }
   if (defer_walks<kCache, kLdsBits>()) {
     const uint64_t tw = a.prof ? wall_clock64() : 0;
@@ -1217,6 +1285,11 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
       }
     }
     asm volatile("" ::: "memory");
+    // Cacheable staging: lines of the state and stamps that this CU's L1 or
+    // this XCD's L2 kept from earlier sweeps are stale now that every
+    // workgroup's (device-coherent, written-through) stores of this sweep
+    // are complete; the acquire drops them before the next sweep stages.
+    if (a.cached_stage) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     changed = __ballot(changed) != 0ull;
     timeout = __ballot(timeout) != 0ull;
     const bool done = !changed || k + 1 >= a.k_end || timeout;
@@ -1253,8 +1326,8 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
         }
         cnt.crit_ticks += maxwork;  // written out once, at the end of the launch
         // Sweep k + 2 appends to list (k + 3) & 3, last read in sweep k - 1;
-        // this store completes before this wave's next barrier arrival.
-        st_relaxed(count_of(a.q, k + 3), 0u);
+        // these stores complete before this wave's next barrier arrival.
+        for (uint32_t x = 0; x < a.q.nsub; ++x) st_relaxed(count_of(a.q, k + 3, x), 0u);
         if (a.prof) {
           s_release = wall_clock64();
           if (row < a.prof_sweeps) {
@@ -1306,7 +1379,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   // writes [k & 1], thread 0 clears it after barrier k, and it is written
   // again only in sweep k + 2, after barrier k + 1.
   __shared__ unsigned s_changed[2];
-  __shared__ unsigned s_next[2];  // entries drawn in sweep k (by parity, like s_changed)
+  __shared__ unsigned s_next[2][kXcds];  // entries drawn from each sub-list in sweep k (by parity, like s_changed)
   __shared__ unsigned long long s_ph[2][4];
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
@@ -1319,7 +1392,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   DeferLds *dq = reinterpret_cast<DeferLds *>(slices + nw) + wave;
   const unsigned lds_off = 0;
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
-  if (threadIdx.x < 2) s_next[threadIdx.x] = 0;
+  if (threadIdx.x < 2 * kXcds) s_next[threadIdx.x / kXcds][threadIdx.x % kXcds] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   if (kLdsBits) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
@@ -1328,7 +1401,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
-    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, &s_next[k & 1], lds_off, k, s, dq, cnt);
+    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, s_next[k & 1], lds_off, k, s, dq, cnt);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
@@ -1343,9 +1416,9 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
     const int done = grid_barrier(a, k, s_changed[k & 1] != 0u, t0, cnt, s_ph[k & 1]);
     if (threadIdx.x == 0) {
       s_changed[k & 1] = 0;
-      s_next[k & 1] = 0;
       t0 = wall_clock64();
     }
+    if (threadIdx.x < kXcds) s_next[k & 1][threadIdx.x] = 0;
     if (threadIdx.x < 4) s_ph[k & 1][threadIdx.x] = 0;
     if (done) break;
   }
@@ -1376,7 +1449,7 @@ __global__ void sweep_book(Args a, int k) {
   Control *c = a.ctl;
   const unsigned ch = c->changes.v[k & 1];
   c->changes.v[(k + 1) & 1] = 0u;
-  *count_of(a.q, k + 3) = 0u;
+  for (uint32_t x = 0; x < a.q.nsub; ++x) *count_of(a.q, k + 3, x) = 0u;
   const bool done = ch == 0u || k + 1 >= a.k_end;
   if (done) {
     c->book.v[4] = static_cast<unsigned>(k);
@@ -1924,6 +1997,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     const dim3 grid(cu * 4);
     launched_groups = grid.x;
     a.lds_bits = 0;
+    a.cached_stage = 0;
     for (int k = L.k_begin;; ++k) {
       if (L.cache)
         hipLaunchKernelGGL(propagate_sweep<true>, grid, dim3(256), 0, stream, a, k);
@@ -1956,6 +2030,9 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     }
     // per-wave LDS: the tile slice, plus the deferred-work block when walks
     // are deferred (line-of-sight flags cached and the bitset read from L2)
+    // With the tiles split by XCD, a scenario's halos are staged through one
+    // L2: stage them with cacheable loads (the barrier then acquires).
+    a.cached_stage = (L.q.nsub > 1 && !a.lds_bits) ? 1 : 0;
     const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(DeferLds) : 0);
     const int max_waves = (a.lds_bits ? persistent_threads<true>() : persistent_threads<false>()) / kWave;
     if (tn.waves_per_group) {
@@ -1974,6 +2051,8 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (groups > max_groups) groups = static_cast<unsigned>(max_groups);
     if (groups > 64u * kWave) groups = 64u * kWave;  // a barrier poller reads <= 64 words per lane
     if (tn.workgroups && tn.workgroups < groups) groups = tn.workgroups;
+    // workgroup w draws from sub-list w % 8: a whole number per XCD
+    if (L.q.nsub > 1 && groups >= kXcds) groups -= groups % kXcds;
     launched_groups = groups;
     if (use_graph) {
       LaunchGraph *lg = nullptr;
@@ -2087,14 +2166,14 @@ struct CallSpace {
 };
 
 // Bytes of the call workspace of one scenario on the raster of sc.
-static size_t call_ws_bytes(const Scen &sc) {
+static size_t call_ws_bytes(const Scen &sc, size_t nsub) {
   const size_t W = static_cast<size_t>(sc.g.W), H = static_cast<size_t>(sc.g.H);
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
   const size_t nwords = size_t{1} << sc.nwords_log2;
   size_t off = align_up(scen_buffer_bytes(W, H), 256);
   off = align_up(off + 2 * nwords * sizeof(uint32_t), 256);
-  off = align_up(off + 4 * ntiles * sizeof(unsigned long long), 256);
-  off = align_up(off + 512, 256);
+  off = align_up(off + 4 * nsub * ntiles * sizeof(unsigned long long), 256);
+  off = align_up(off + 512 * nsub, 256);
   off = align_up(off + sizeof(Scen), 256);
   off = align_up(off + sizeof(ScenOut), 256);
   return off;
@@ -2122,11 +2201,12 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
   const size_t o_scen = 0;
   const size_t o_mark = align_up(o_scen + scen_buffer_bytes(W, H), 256);
+  const size_t nsub = kXcds;  // tile lists split by XCD (see TileQueue)
   const size_t o_list = align_up(o_mark + 2 * nwords * sizeof(uint32_t), 256);
-  const size_t o_count = align_up(o_list + 4 * ntiles * sizeof(unsigned long long), 256);
-  const size_t o_desc = align_up(o_count + 512, 256);
+  const size_t o_count = align_up(o_list + 4 * nsub * ntiles * sizeof(unsigned long long), 256);
+  const size_t o_desc = align_up(o_count + 512 * nsub, 256);
   const size_t o_out = align_up(o_desc + sizeof(Scen), 256);
-  const size_t need = call_ws_bytes(sc);
+  const size_t need = call_ws_bytes(sc, nsub);
   char *base = nullptr;
   std::unique_lock<std::mutex> lk;
   if (space) {
@@ -2155,6 +2235,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   q.list = reinterpret_cast<unsigned long long *>(base + o_list);
   q.count = reinterpret_cast<uint32_t *>(base + o_count);
   q.cap = ntiles;
+  q.nsub = static_cast<uint32_t>(nsub);
   Scen *d_scen = reinterpret_cast<Scen *>(base + o_desc);
   ScenOut *d_out = reinterpret_cast<ScenOut *>(base + o_out);
 
@@ -2173,11 +2254,11 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   } guard{{&ev0, &ev1}};
   if (ev0) PG_HIP(hipEventRecord(ev0, stream));
   const unsigned cu = static_cast<unsigned>(dev->cu_count);
-  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * ntiles + 255) / 256, cu * 8));
+  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * nsub * ntiles + 255) / 256, cu * 8));
   hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, sc.mark[0], static_cast<size_t>(2 * nwords), 0u);
   hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t *>(q.list), 8 * ntiles, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count, static_cast<size_t>(128), 0u);
+                     reinterpret_cast<uint32_t *>(q.list), 8 * nsub * ntiles, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count, static_cast<size_t>(128 * nsub), 0u);
   PG_HIP(hipGetLastError());
   st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, true, stream);
   if (st != PG_OK) return st;

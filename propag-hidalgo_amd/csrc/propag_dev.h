@@ -16,8 +16,8 @@ constexpr int kTile = 8;  // tile edge (cells): one cell per lane of a wave
 
 // Bytes of the device array behind pg_propagate_noalloc: the call and launch
 // workspaces of any raster of at most PG_NOALLOC_MAX_CELLS cells (the largest,
-// a 17 x 61680 raster, needs 38.5 MiB on a 256-CU device).
-constexpr size_t kNoAllocScratchBytes = size_t{40} << 20;
+// a 17 x 61680 raster, needs 43.5 MiB on a 256-CU device).
+constexpr size_t kNoAllocScratchBytes = size_t{44} << 20;
 
 struct Geom {
   int W, H;
@@ -61,17 +61,24 @@ struct ScenOut {
 };
 
 // Per-sweep queues of tiles to visit.  The tiles marked for sweep k sit in
-// list[k & 3][0 .. count[k & 3]) as scenario << 32 | tile, each once (the
-// scenario's mark bitmap of parity k & 1 has the tile's bit set by the first
-// marker; the reader clears it).  Unused list slots hold ~0.  Four lists
-// rotate so that a count can be reset two barriers before it is appended
-// to again.  The queues outlive a launch: a launch that stops at k_end
-// leaves the tiles of sweep k_end queued.
+// the nsub sub-lists of list k & 3, sub-list x holding
+// list[k & 3][x][0 .. count[k & 3][x]) as scenario << 32 | xcd << 29 | tile,
+// each tile once (the scenario's mark bitmap of parity k & 1 has the tile's
+// bit set by the first marker; the reader clears it).  With nsub = 8 a
+// tile goes to the sub-list of its xcd tag and is run by the workgroups of
+// that XCD (workgroup w runs on XCD w % 8); with nsub = 1 every workgroup
+// draws from the one list.  Unused list slots hold ~0.  Four lists rotate so
+// that a count can be reset two barriers before it is appended to again.
+// The queues outlive a launch: a launch that stops at k_end leaves the
+// tiles of sweep k_end queued.
 struct TileQueue {
-  unsigned long long *list;  // list p at list + p * cap
-  uint32_t *count;           // count p at count + 32 * p (one 128-byte line each)
-  size_t cap;                // slots per list (>= tiles of all scenarios)
+  unsigned long long *list;  // sub-list (p, x) at list + (p * nsub + x) * cap
+  uint32_t *count;           // its count at count + 32 * (p * nsub + x) (one 128-byte line each)
+  size_t cap;                // slots per sub-list (>= tiles of all scenarios)
+  uint32_t nsub;             // sub-lists per list: 1, or 8 (one per XCD)
+  uint32_t pad;
 };
+constexpr uint32_t kXcds = 8;  // XCDs of an MI355X; workgroup w runs on XCD w % 8
 
 // Bitmap words a scenario of ntiles tiles uses (a power of two >= ntiles/32).
 inline uint32_t need_words_log2(size_t ntiles) {

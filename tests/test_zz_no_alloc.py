@@ -45,7 +45,10 @@ def _run(sc, fire, noalloc, tuning=None):
 def _same(a, b):
     for k in ("time", "refs_x", "refs_y", "boundaries"):
         assert np.array_equal(a[k], b[k]), k
-    for k in ("iterations", "relaxations", "improvements", "dda_steps", "tiles_visited"):
+    # sweeps and commits are fixed by the exact schedule; how many cells are
+    # evaluated is not (a stamp written in the same sweep may or may not be
+    # seen, which only adds evaluations that commit nothing)
+    for k in ("iterations", "improvements"):
         assert a["stats"][k] == b["stats"][k], k
 
 
