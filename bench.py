@@ -342,7 +342,8 @@ def main():
     fuel = torch.from_numpy(land["fuel"]).to(dev)
     slope = torch.from_numpy(land["slope"]).to(dev)
     aspect = torch.from_numpy(land["aspect"]).to(dev)
-    fire = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    fire_all = torch.zeros(3 * n, dtype=torch.float32, device=dev)   # one fill pre-zeroes all three
+    fire = [fire_all[i * n:(i + 1) * n] for i in range(3)]
     t_buf = torch.empty(n, dtype=torch.float32, device=dev)
     rx_buf = torch.empty(n, dtype=torch.int16, device=dev)
     ry_buf = torch.empty(n, dtype=torch.int16, device=dev)
@@ -364,8 +365,7 @@ def main():
         t_buf.copy_(t0_)
         rx_buf.copy_(rx0)
         ry_buf.copy_(ry0)
-        for f in fire:
-            f.zero_()
+        fire_all.zero_()
         pp.burn_device(fuel, *cols, *fire, stream=stream)
         return pp.propagate_device(st, *fire, t_buf, rx_buf, ry_buf, bits, tuning=tuning, stream=stream)
 

@@ -1591,9 +1591,14 @@ __global__ __launch_bounds__(256) void boundaries_kernel(Geom g, const float *sp
 }
 
 // post_propagate (propag.cu:110-117) plus unpacking into the caller's rasters.
-__global__ void post(Scen sc, int final_buf, const float *speed, float *time, uint16_t *rx, uint16_t *ry) {
+__global__ void post(Scen sc, int final_buf, const ScenOut *out, const float *speed, float *time, uint16_t *rx,
+                     uint16_t *ry) {
   const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
-  const uint2 *st = sc.st[final_buf];
+  if (out) {
+    const ScenOut o = *out;
+    final_buf = scen_final_buf(sc, o);
+  }
+  const uint2 *st = final_buf ? sc.st[1] : sc.st[0];
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const uint2 w = st[i];
@@ -1887,12 +1892,12 @@ pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream) {
 }
 
 pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,
-                      uint16_t *refs_y, hipStream_t stream) {
+                      uint16_t *refs_y, hipStream_t stream, const ScenOut *out) {
   const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
   size_t b = (n + 255) / 256;
   if (b > 2048) b = 2048;
-  hipLaunchKernelGGL(post, dim3(static_cast<unsigned>(b)), dim3(256), 0, stream, sc, final_buf, speed, time, refs_x,
-                     refs_y);
+  hipLaunchKernelGGL(post, dim3(static_cast<unsigned>(b)), dim3(256), 0, stream, sc, final_buf, out, speed, time,
+                     refs_x, refs_y);
   PG_HIP(hipGetLastError());
   return PG_OK;
 }
@@ -2012,6 +2017,18 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (evr1) PG_HIP(hipEventRecord(evr1, stream));
     return PG_OK;
   };
+  // After the relaxation: the caller's follow-up work, the control block
+  // (books and counters) to the host, one synchronisation.
+  Control hc{};
+  auto finish_launch = [&](bool copy_ctl) -> pg_status {
+    if (L.after) {
+      const pg_status s2 = L.after(L.after_ctx, stream);
+      if (s2 != PG_OK) return s2;
+    }
+    if (copy_ctl) PG_HIP(hipMemcpyAsync(&hc, a.ctl, sizeof(Control), hipMemcpyDeviceToHost, stream));
+    PG_HIP(hipStreamSynchronize(stream));
+    return PG_OK;
+  };
   if (tn.mode == 1) {
     st = run_per_sweep();
     if (st != PG_OK) return st;
@@ -2069,7 +2086,8 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       const hipError_t ge = hipGraphLaunch(lg->exec, stream);
       if (ge != hipErrorIllegalState && ge != hipErrorCooperativeLaunchTooLarge) {
         PG_HIP(ge);
-        PG_HIP(hipStreamSynchronize(stream));
+        st = finish_launch(false);  // the graph copies the control block itself
+        if (st != PG_OK) return st;
         std::memcpy(book, &lg->host->book, sizeof(book));
         unsigned long long counters[16];
         std::memcpy(counters, lg->host->stats, sizeof(counters));
@@ -2101,17 +2119,15 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       } else {
         PG_HIP(le);
         if (evr1) PG_HIP(hipEventRecord(evr1, stream));
-        PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
-        PG_HIP(hipStreamSynchronize(stream));
       }
     }
   }
-  unsigned long long counters[16];
-  PG_HIP(hipMemcpyAsync(counters, a.ctl->stats, sizeof(counters), hipMemcpyDeviceToHost, stream));
-  PG_HIP(hipStreamSynchronize(stream));
+  st = finish_launch(true);
+  if (st != PG_OK) return st;
+  std::memcpy(book, hc.book.v, sizeof(book));
   float rms = 0.0f;
   if (evr0) PG_HIP(hipEventElapsedTime(&rms, evr0, evr1));
-  return finish_stats(dev, book, counters, rms, launched_groups, a, prof_rows, stats, res);
+  return finish_stats(dev, book, hc.stats, rms, launched_groups, a, prof_rows, stats, res);
 }
 
 // Launch outcome and counters into res / stats.
@@ -2260,9 +2276,21 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
                      reinterpret_cast<uint32_t *>(q.list), 8 * nsub * ntiles, 0xFFFFFFFFu);
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count, static_cast<size_t>(128 * nsub), 0u);
   PG_HIP(hipGetLastError());
-  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, true, stream);
+  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, false, stream);
   if (st != PG_OK) return st;
   PG_HIP(hipMemcpyAsync(d_scen, &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
+  // Post-pass and outcome right behind the relaxation, so the call waits on
+  // its stream once: the post kernel reads the outcome on the device.
+  struct After {
+    const Scen *sc;
+    ScenOut *d_out, *h_out;
+    const float *speed;
+    float *time;
+    uint16_t *rx, *ry;
+    hipEvent_t ev1;
+  };
+  ScenOut o{};
+  After aft{&sc, d_out, &o, speed, time, refs_x, refs_y, ev1};
 
   Launch L{};
   L.scen = d_scen;
@@ -2276,6 +2304,15 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   L.lds_bits_words = (W * H + 31) / 32;
   L.ws = space ? space->launch : nullptr;
   L.untimed = timed ? 0 : 1;
+  L.after = [](void *p, hipStream_t s) -> pg_status {
+    const After &f = *static_cast<const After *>(p);
+    const pg_status fs = scen_finish(*f.sc, 0, f.speed, f.time, f.rx, f.ry, s, f.d_out);
+    if (fs != PG_OK) return fs;
+    PG_HIP(hipMemcpyAsync(f.h_out, f.d_out, sizeof(ScenOut), hipMemcpyDeviceToHost, s));
+    if (f.ev1) PG_HIP(hipEventRecord(f.ev1, s));
+    return PG_OK;
+  };
+  L.after_ctx = &aft;
   pg_stats local{};
   LaunchResult res{};
   st = run_sweeps(dev, L, &tn, &local, &res, stream);
@@ -2284,13 +2321,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
     (void)hipStreamSynchronize(stream);
     return st;
   }
-  ScenOut o{};
-  PG_HIP(hipMemcpyAsync(&o, d_out, sizeof(o), hipMemcpyDeviceToHost, stream));
-  PG_HIP(hipStreamSynchronize(stream));
-  st = scen_finish(sc, scen_final_buf(sc, o), speed, time, refs_x, refs_y, stream);
-  if (st != PG_OK) return st;
-  if (ev1) PG_HIP(hipEventRecord(ev1, stream));
-  PG_HIP(hipStreamSynchronize(stream));
+  if (o.err & 2u) return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
   float ms = 0.0f;
   if (ev0) PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
   if (stats) {

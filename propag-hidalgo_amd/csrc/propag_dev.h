@@ -120,12 +120,13 @@ pg_status scen_init_state(float *time, uint16_t *refs_x, uint16_t *refs_y, size_
 
 pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream);
 
-// post_propagate for one scenario from state buffer `final_buf`.
+// post_propagate for one scenario from state buffer `final_buf`, or, when
+// `out` (device) is given, from the buffer its launch outcome names.
 pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,
-                      uint16_t *refs_y, hipStream_t stream);
+                      uint16_t *refs_y, hipStream_t stream, const ScenOut *out = nullptr);
 
 // Buffer that holds a finished scenario's final state.
-inline int scen_final_buf(const Scen &sc, const ScenOut &o) {
+__host__ __device__ inline int scen_final_buf(const Scen &sc, const ScenOut &o) {
   if (o.capped) return static_cast<int>((static_cast<unsigned>(sc.k0) + sc.max_iters) & 1u);
   return (o.last_change + 2) & 1;
 }
@@ -144,6 +145,11 @@ struct Launch {
   char *ws;            // launch workspace of launch_ws_bytes(cu count) bytes, or null for the
                        // per-device one (grown with hipMalloc)
   int untimed;         // 1: no HIP events (relax_ms stays 0)
+  // Work to enqueue on the stream right after the relaxation kernel(s)
+  // (before the launch's one synchronisation), e.g. the post-pass; it
+  // returns PG_OK or an error.  May be empty.
+  pg_status (*after)(void *ctx, hipStream_t stream);
+  void *after_ctx;
 };
 
 // Bytes of a launch workspace (control block, barrier words, profile rows
