@@ -49,6 +49,15 @@ FIELDS = ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed", "wind_azimuth
 TRAFFIC_PROFILE = "profiles/propagate_batched_ensemble_1024_hbm_traffic.md"
 
 
+def roofline(relaxations, improvements, kernel_ms):
+    """HBM roofline of a relaxation kernel: algorithmic bytes (24 B per cell
+    relaxation + 8 B per committed improvement) over its HIP-event time."""
+    alg = 24.0 * relaxations + 8.0 * improvements
+    gbs = alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "algorithmic_bytes": alg, "kernel_ms": kernel_ms}
+
+
 def measured_traffic():
     """HBM bytes per launch of the batched kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters cannot be read from
@@ -152,9 +161,10 @@ def large_single(args, pp, synth, torch, tuning, stream):
            "sweeps": st["iterations"], "relaxations": st["relaxations"], "improvements": st["improvements"],
            "tiles_visited": st["tiles_visited"], "dda_steps": st["dda_steps"], "burned_cells": burned,
            "repeatable": bool(same), "ignitions": len(sc["ignitions"])}
+    out["roofline"] = roofline(out["relaxations"], out["improvements"], out["relax_ms"])
     log(f"large 4096^2: {out['ms']:.2f} ms per scenario (relax {out['relax_ms']:.2f} ms), {out['sweeps']} sweeps, "
         f"{burned} burned, {out['tiles_visited']} tile visits; repeatable {same}")
-    return out
+    return out, (sc, t.cpu().numpy())
 
 
 def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
@@ -216,13 +226,15 @@ def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
                        "log-uniform 3600-43200 s, 1-4 ignitions, uniform wind U(2,12) m/s",
            "value": value, "unit": "scenarios/s", "ms_per_ensemble": float(np.mean(ms)),
            "relax_ms": stats["relax_ms"], "sweeps": stats["iterations"], "tiles_visited": stats["tiles_visited"],
-           "relaxations": stats["relaxations"], "dda_steps": stats["dda_steps"],
+           "relaxations": stats["relaxations"], "improvements": stats["improvements"], "dda_steps": stats["dda_steps"],
+           "roofline": roofline(stats["relaxations"], stats["improvements"], stats["relax_ms"]),
            "plain_value": plain_sps, "plain_sample": len(plain_ms), "speedup_batched": value / plain_sps,
            "parity_first_scenarios": parity, "autotune": tune}
     log(f"ensemble 2048: {out['ms_per_ensemble']:.1f} ms per ensemble of {len(wl['cases'])} ({value:.1f} "
-        f"scenarios/s; one per call {plain_sps:.1f}/s, {value / plain_sps:.1f}x), {stats['iterations']} sweeps, "
+        f"scenarios/s; one per call {plain_sps:.1f}/s over {len(plain_ms) - 1} timed calls, "
+        f"{value / plain_sps:.1f}x), relax kernel {stats['relax_ms']:.1f} ms, {stats['iterations']} sweeps, "
         f"parity {parity}")
-    return out
+    return out, wl
 
 
 def serve_stream(args, pp, synth, torch, tuning):
@@ -291,6 +303,133 @@ def serve_stream(args, pp, synth, torch, tuning):
     return out
 
 
+def readme_section(pp, synth, torch, tuning, stream, calls=10):
+    """The reference README's CLI scenario (crates/propag/src/main.rs: 1024 x
+    1024 at 5 m, fuel model 1, all moistures 0.1, wind 5 m/s toward azimuth
+    0, flat, ignition at (500, 500), 10 h), timed per call in the drop-in
+    form (pg_burn + pg_propagate on resident inputs).  Context for the one
+    figure the reference publishes, README.md:98-108's "10 loops: 79.46 ms"
+    on an unnamed 128-SM NVIDIA GPU from an older build (whether it covers
+    one run or ten is unclear) - not a target."""
+    dev = torch.device("cuda:0")
+    sc = synth.uniform_scenario(side=1024)
+    n = sc["width"] * sc["height"]
+    g = pp.GeoReference(sc["width"], sc["height"], "+proj=utm +zone=30", [0.0, sc["dx"], 0.0, 0.0, 0.0, sc["dy"]])
+    settings = pp.Settings(g, float(sc["max_time"]))
+    fuel = torch.from_numpy(sc["terrain"]["fuel"]).to(dev)
+    cols = device_columns(torch, dev, sc["terrain"], n, FIELDS)
+    init = (torch.from_numpy(sc["time"]).to(dev), torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev),
+            torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev))
+    plain_run(pp, torch, dev, stream, settings, fuel, cols, init, tuning)
+    runs = [plain_run(pp, torch, dev, stream, settings, fuel, cols, init, tuning) for _ in range(calls)]
+    ms = [r[0] for r in runs]
+    t = runs[-1][2]
+    burned = int((t < pp.FLT_MAX).sum().item())
+    out = {"workload": "reference README CLI scenario (main.rs): 1024x1024 at 5 m, fuel 1, moisture 0.1, "
+                       "wind 5 m/s toward 0, flat, ignition (500,500), 10 h; pg_burn + pg_propagate per call",
+           "ms": float(np.mean(ms)), "p50_ms": float(np.percentile(ms, 50)), "calls": calls,
+           "relax_ms": float(np.mean([r[1]["relax_ms"] for r in runs])), "sweeps": runs[-1][1]["iterations"],
+           "burned_cells": burned, "max_time_s": float(t[t < pp.FLT_MAX].max().item()),
+           "reference_published_ms": 79.46, "reference_burned_cells": 447822,
+           "reference_hardware": "unnamed NVIDIA GPU, 128 SMs, older build (README.md:98-112)",
+           "note": "context only: the reference's figure is from another GPU and build"}
+    log(f"README scenario: {out['ms']:.2f} ms per call (relax {out['relax_ms']:.2f} ms), {out['sweeps']} sweeps, "
+        f"{burned} cells burned; the reference README printed 79.46 ms and 447822 cells")
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_section(args, sc, field, large=None, ens2048=None):
+    """The oracle on the host cores, beside the GPU figures (a reported
+    baseline, not a target).  The oracle relaxes every cell of the burned
+    bounding box (grown by one) in every sweep and runs the same sweeps as
+    the GPU (the schedule is the same exact Jacobi)."""
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    out = {"host": {"cpu_model": cpu_model(), "cores_available": cores, "nproc": os.cpu_count()}}
+    W, H = sc["width"], sc["height"]
+    # config-1: burn + serial propagation to convergence, one thread
+    t0 = time.perf_counter()
+    fire = oracle.burn_f32(sc["terrain"], nthreads=1)
+    og = oracle.grid(W, H, sc["dx"], sc["dy"], sc["max_time"])
+    ref = oracle.propagate(og, *fire, sc["time"], sc["refs_x"], sc["refs_y"], nthreads=1)
+    c1_s = time.perf_counter() - t0
+    out["config1"] = {"value": 1.0 / c1_s, "unit": "scenarios/s", "cores": 1, "kind": "port",
+                      "sample": "one full config-1 scenario (burn + serial exact-Jacobi propagation to "
+                                f"convergence, {ref['sweeps']} sweeps) on 1 thread, {c1_s:.2f} s",
+                      "seconds": c1_s, "sweeps": ref["sweeps"], "relaxations": ref["relaxations"]}
+    log(f"cpu config-1: {c1_s:.2f} s per scenario (1 thread, {ref['sweeps']} sweeps)")
+    # Stage F on every host core (after one untimed call that starts the threads)
+    oracle.burn_f32(field, nthreads=cores)
+    t1 = time.perf_counter()
+    oracle.burn_f32(field, nthreads=cores)
+    burn_s = time.perf_counter() - t1
+    n0 = field["fuel"].size
+    out["rothermel"] = {"value": n0 / burn_s / 1e6, "unit": "Mcells/s", "cores": cores, "kind": "port",
+                        "sample": f"the 1024x1024 i.i.d. field, f32 oracle burn on {cores} threads, {burn_s * 1e3:.1f} ms"}
+    log(f"cpu rothermel: {n0 / burn_s / 1e6:.1f} Mcells/s on {cores} threads ({cpu_model()})")
+    # 4096^2 (C2): first sweeps on one thread, extrapolated
+    if large is not None:
+        lsc, lres = large
+        K = args.cpu_c2_sweeps
+        LW, LH = lsc["width"], lsc["height"]
+        lfire = oracle.burn_f32(lsc["terrain"], nthreads=cores)
+        lg = oracle.grid(LW, LH, lsc["dx"], lsc["dy"], lsc["max_time"])
+        t2 = time.perf_counter()
+        part = oracle.propagate(lg, *lfire, lsc["time"], lsc["refs_x"], lsc["refs_y"], nthreads=1, max_sweeps=K)
+        part_s = time.perf_counter() - t2
+        burned = np.nonzero(lres["time_host"] < oracle.FLT_MAX)[0]
+        bx, by = burned % LW, burned // LW
+        box = (min(int(bx.max()) + 1, LW - 1) - max(int(bx.min()) - 1, 0) + 1) * \
+              (min(int(by.max()) + 1, LH - 1) - max(int(by.min()) - 1, 0) + 1)
+        per_relax = part_s / max(1, part["relaxations"])
+        est = per_relax * box * lres["sweeps"]
+        out["large_4096"] = {"value": 1.0 / est, "unit": "scenarios/s", "cores": 1, "kind": "port",
+                             "estimated": True, "seconds_est": est,
+                             "sample": f"the first {K} of {lres['sweeps']} sweeps on 1 thread ({part_s:.2f} s, "
+                                       f"{part['relaxations']} cell relaxations); full run estimated as the "
+                                       "time per relaxation x the final burned box grown by one cell x the "
+                                       "GPU's sweep count (an upper bound: the box only grows)"}
+        log(f"cpu 4096^2: first {K} sweeps {part_s:.2f} s on 1 thread; full run est {est:.0f} s")
+    # 2048^2 ensemble (C3): a seeded sample of its scenarios, all cores
+    if ens2048 is not None:
+        side, land, cases = ens2048["side"], ens2048["land"], ens2048["cases"]
+        done, spent = 0, 0.0
+        for c in cases:
+            if done >= args.cpu_c3_sample or (done > 0 and spent > args.cpu_c3_seconds):
+                break
+            terr = {"fuel": land["fuel"], "slope": land["slope"], "aspect": land["aspect"]}
+            for k in FIELDS[:7]:
+                terr[k] = np.full(side * side, np.float32(c["wx"][k]), np.float32)
+            from py_propag import synth
+            t3 = time.perf_counter()
+            f3 = oracle.burn_f32(terr, nthreads=cores)
+            tm, rx, ry = synth.initial_state(side, side, c["ign"])
+            oracle.propagate(oracle.grid(side, side, 5.0, 5.0, c["max_time"]), *f3, tm, rx, ry, nthreads=cores)
+            spent += time.perf_counter() - t3
+            done += 1
+        out["ensemble_2048"] = {"value": done / spent, "unit": "scenarios/s", "cores": cores, "kind": "port",
+                                "sample": f"the first {done} scenarios of the seeded ensemble (burn + propagation "
+                                          f"to convergence, the oracle on {cores} threads), {spent:.2f} s"}
+        log(f"cpu 2048^2 ensemble: {done} scenarios in {spent:.2f} s on {cores} threads")
+    return out
+
+
+SECTIONS = ("config1", "plain", "ensemble", "serving", "ensemble2048", "large4096", "readme", "rothermel",
+            "moisture", "cpu")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,8 +448,13 @@ def main():
                     help="length of the served Poisson request stream (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="request arrivals per second")
     ap.add_argument("--batch2048", type=int, default=256, help="scenarios of the 2048x2048 ensemble (0: skip)")
-    ap.add_argument("--plain2048", type=int, default=6, help="of which timed one per call")
+    ap.add_argument("--plain2048", type=int, default=17, help="of which timed one per call (the first is untimed)")
     ap.add_argument("--no-large", action="store_true", help="skip the 4096x4096 scenario")
+    ap.add_argument("--cpu-c2-sweeps", type=int, default=12, help="sweeps of the 4096^2 CPU sample")
+    ap.add_argument("--cpu-c3-sample", type=int, default=4, help="scenarios of the 2048^2 CPU sample")
+    ap.add_argument("--cpu-c3-seconds", type=float, default=20.0, help="time budget of the 2048^2 CPU sample")
+    ap.add_argument("--only", choices=SECTIONS, default=None,
+                    help="run one section alone and print its result (profiling; not the benchmark line)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (rocprof counter scaling)")
     args = ap.parse_args()
@@ -318,6 +462,7 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
               "scenarios are independent and shard one-per-GPU, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    want = (lambda name: True) if args.only is None else (lambda name: name == args.only)
 
     import torch
     import py_propag as pp
@@ -369,129 +514,166 @@ def main():
         pp.burn_device(fuel, *cols, *fire, stream=stream)
         return pp.propagate_device(st, *fire, t_buf, rx_buf, ry_buf, bits, tuning=tuning, stream=stream)
 
+    results = {}
+
     # ------------------------------------------- plain: config-1 scenario
-    c1_cols = [torch.from_numpy(np.ascontiguousarray(sc["terrain"][k], np.float32)).to(dev) for k in FIELDS]
-    c1_init = (torch.from_numpy(sc["time"]).to(dev), torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev),
-               torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev))
-    for _ in range(2):
-        plain_step(c1_cols, *c1_init)
-    torch.cuda.synchronize()
-    c1_lat, c1_stats = [], []
-    for _ in range(10):
-        s0 = time.perf_counter()
-        c1_stats.append(plain_step(c1_cols, *c1_init))
-        c1_lat.append((time.perf_counter() - s0) * 1e3)
-    burned = int((t_buf < pp.FLT_MAX).sum().item())
-    st1 = c1_stats[-1]
-    relax_ms = float(np.mean([s["relax_ms"] for s in c1_stats]))
-    log(f"config-1 plain: {np.mean(c1_lat):.3f} ms/scenario, relax kernel {relax_ms:.3f} ms, "
-        f"{st1['iterations']} sweeps, {st1['relaxations']} relaxations, {st1['tiles_visited']} tile visits, "
-        f"{burned} burned; critical path {st1['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) "
-        f"{st1['phase_ms']}, barrier (start/span/gather/book) {st1['barrier_ms']}")
+    if want("config1"):
+        c1_cols = [torch.from_numpy(np.ascontiguousarray(sc["terrain"][k], np.float32)).to(dev) for k in FIELDS]
+        c1_init = (torch.from_numpy(sc["time"]).to(dev), torch.from_numpy(sc["refs_x"].view(np.int16)).to(dev),
+                   torch.from_numpy(sc["refs_y"].view(np.int16)).to(dev))
+        for _ in range(2):
+            plain_step(c1_cols, *c1_init)
+        torch.cuda.synchronize()
+        c1_lat, c1_stats = [], []
+        for _ in range(10):
+            s0 = time.perf_counter()
+            c1_stats.append(plain_step(c1_cols, *c1_init))
+            c1_lat.append((time.perf_counter() - s0) * 1e3)
+        burned = int((t_buf < pp.FLT_MAX).sum().item())
+        st1 = c1_stats[-1]
+        relax_ms = float(np.mean([s["relax_ms"] for s in c1_stats]))
+        log(f"config-1 plain: {np.mean(c1_lat):.3f} ms/scenario, relax kernel {relax_ms:.3f} ms, "
+            f"{st1['iterations']} sweeps, {st1['relaxations']} relaxations, {st1['tiles_visited']} tile visits, "
+            f"{burned} burned; critical path {st1['crit_ms']:.3f} ms, phases (stage/rule/walk/tail) "
+            f"{st1['phase_ms']}, barrier (start/span/gather/book) {st1['barrier_ms']}")
+        results["config1"] = {
+            "ms": float(np.mean(c1_lat)), "p50_ms": float(np.percentile(c1_lat, 50)),
+            "p99_ms": float(np.percentile(c1_lat, 99)),
+            "stats": {"sweeps": st1["iterations"], "relaxations": st1["relaxations"],
+                      "improvements": st1["improvements"], "dda_steps": st1["dda_steps"],
+                      "tiles_visited": st1["tiles_visited"], "burned_cells": burned,
+                      "relax_ms": relax_ms, "crit_ms": st1["crit_ms"],
+                      "phase_ms": st1["phase_ms"], "barrier_ms": st1["barrier_ms"],
+                      "lds_bitset": st1["lds_bitset"], "workgroups": st1["workgroups"]}}
 
     # ------------------------------------------ plain: ensemble scenarios
-    n_plain = min(args.plain, args.batch)
-    res_cases = [resident_case(c) for c in cases[:n_plain]]
-    plain_step(*res_cases[0])
-    torch.cuda.synchronize()
-    plain_lat, plain_out = [], []
-    for i, rc in enumerate(res_cases):
-        s0 = time.perf_counter()
-        plain_step(*rc)
-        plain_lat.append((time.perf_counter() - s0) * 1e3)
-        if i < 2:
-            plain_out.append(t_buf.cpu().numpy().copy())
-    plain_sps = 1e3 / float(np.mean(plain_lat))
-    log(f"ensemble scenarios one per call: {np.mean(plain_lat):.3f} ms/scenario ({plain_sps:.1f} scenarios/s), "
-        f"p50 {np.percentile(plain_lat, 50):.3f} p99 {np.percentile(plain_lat, 99):.3f} ms")
-    del res_cases
+    plain_out = []
+    if want("plain") or want("ensemble"):
+        n_plain = min(args.plain, args.batch)
+        res_cases = [resident_case(c) for c in cases[:n_plain]]
+        plain_step(*res_cases[0])
+        torch.cuda.synchronize()
+        plain_lat = []
+        for i, rc in enumerate(res_cases):
+            s0 = time.perf_counter()
+            plain_step(*rc)
+            plain_lat.append((time.perf_counter() - s0) * 1e3)
+            if i < 2:
+                plain_out.append(t_buf.cpu().numpy().copy())
+        plain_sps = 1e3 / float(np.mean(plain_lat))
+        log(f"ensemble scenarios one per call: {np.mean(plain_lat):.3f} ms/scenario ({plain_sps:.1f} scenarios/s), "
+            f"p50 {np.percentile(plain_lat, 50):.3f} p99 {np.percentile(plain_lat, 99):.3f} ms")
+        results["plain"] = {"value": plain_sps, "lat": plain_lat, "scenarios": n_plain}
+        del res_cases
 
     # ------------------------------------------------ batched: ensembles
-    ctx = pp.Context()
-    tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
-    btuning, tune_1024 = autotuned(args, ctx, tid, cases, W, H, tuning)
+    if want("ensemble"):
+        ctx = pp.Context()
+        tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+        btuning, tune_1024 = autotuned(args, ctx, tid, cases, W, H, tuning)
 
-    def ensemble():
-        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
-        if any(i is None for i in ids):
-            raise RuntimeError("ensemble does not fit the memory budget")
-        fin, stats = ctx.run(tuning=btuning)
-        return ids, fin, stats
+        def ensemble():
+            ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+            if any(i is None for i in ids):
+                raise RuntimeError("ensemble does not fit the memory budget")
+            fin, stats = ctx.run(tuning=btuning)
+            return ids, fin, stats
 
-    for _ in range(args.warmup):
-        ids, _, _ = ensemble()
-        for i in ids:
-            ctx.release(i)
-    torch.cuda.synchronize()
-    ens_ms, ens_stats = [], []
-    wall0 = time.perf_counter()
-    for k in range(args.steps):
-        s0 = time.perf_counter()
-        ids, fin, stats = ensemble()
-        ens_ms.append((time.perf_counter() - s0) * 1e3)
-        ens_stats.append(stats)
-        if fin != len(cases):
-            raise RuntimeError(f"{len(cases) - fin} scenarios did not finish")
-        if k == args.steps - 1:
-            parity = all(np.array_equal(ctx.result(ids[i])[0].ravel().view(np.uint32), plain_out[i].view(np.uint32))
-                         for i in range(len(plain_out)))
-        for i in ids:
-            ctx.release(i)
-    wall = time.perf_counter() - wall0
-    ctx.close()
-    value = args.batch * args.steps / wall
-    es = ens_stats[-1]
-    log(f"batched: {np.mean(ens_ms):.1f} ms per ensemble of {args.batch} ({value:.1f} scenarios/s, "
-        f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
-        f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle), {es['dda_steps']} walk steps; "
-        f"parity with one-per-call: {parity}; phases (stage/rule/walk/tail) {es['phase_ms']}, "
-        f"phase totals over waves {es['phase_total_ms']}, "
-        f"barrier (start/span/gather/book) {es['barrier_ms']}; per workgroup: work {es['work_ms']:.1f} ms, "
-        f"wait {es['wait_ms']:.1f} ms, slowest-workgroup work summed over sweeps {es['crit_ms']:.1f} ms")
-    serving = serve_stream(args, pp, synth, torch, tuning) if args.serve_seconds > 0 else None
-    ens2048 = ensemble_2048_section(args, pp, synth, torch, tuning, stream) if args.batch2048 > 0 else None
-    large = None if args.no_large else large_single(args, pp, synth, torch, tuning, stream)
-    ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
-    alg_bytes = 24.0 * es["relaxations"] + 8.0 * es["improvements"]
-    achieved = alg_bytes / (ens_relax_ms * 1e-3) / 1e9
+        for _ in range(args.warmup):
+            ids, _, _ = ensemble()
+            for i in ids:
+                ctx.release(i)
+        torch.cuda.synchronize()
+        ens_ms, ens_stats = [], []
+        wall0 = time.perf_counter()
+        for k in range(args.steps):
+            s0 = time.perf_counter()
+            ids, fin, stats = ensemble()
+            ens_ms.append((time.perf_counter() - s0) * 1e3)
+            ens_stats.append(stats)
+            if fin != len(cases):
+                raise RuntimeError(f"{len(cases) - fin} scenarios did not finish")
+            if k == args.steps - 1:
+                parity = all(np.array_equal(ctx.result(ids[i])[0].ravel().view(np.uint32),
+                                            plain_out[i].view(np.uint32)) for i in range(len(plain_out)))
+            for i in ids:
+                ctx.release(i)
+        wall = time.perf_counter() - wall0
+        ctx.close()
+        value = args.batch * args.steps / wall
+        es = ens_stats[-1]
+        plain_sps = results["plain"]["value"]
+        log(f"batched: {np.mean(ens_ms):.1f} ms per ensemble of {args.batch} ({value:.1f} scenarios/s, "
+            f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
+            f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle), "
+            f"{es['dda_steps']} walk steps; parity with one-per-call: {parity}; phases (stage/rule/walk/tail) "
+            f"{es['phase_ms']}, phase totals over waves {es['phase_total_ms']}, "
+            f"barrier (start/span/gather/book) {es['barrier_ms']}; per workgroup: work {es['work_ms']:.1f} ms, "
+            f"wait {es['wait_ms']:.1f} ms, slowest-workgroup work summed over sweeps {es['crit_ms']:.1f} ms")
+        ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
+        rl = roofline(es["relaxations"], es["improvements"], ens_relax_ms)
+        rl.update({"traffic": measured_traffic(), "traffic_source": TRAFFIC_PROFILE,
+                   "kernel": "propagate_sweep" if args.mode == 1 else "propagate_persistent"})
+        results["ensemble"] = {"value": value, "wall": wall, "ens_ms": ens_ms, "stats": es, "parity": parity,
+                               "tuning": btuning, "autotune": tune_1024, "roofline": rl}
+
+    serving = serve_stream(args, pp, synth, torch, tuning) if want("serving") and args.serve_seconds > 0 else None
+    ens2048, wl2048 = (ensemble_2048_section(args, pp, synth, torch, tuning, stream)
+                       if want("ensemble2048") and args.batch2048 > 0 else (None, None))
+    large, large_sc = (large_single(args, pp, synth, torch, tuning, stream)
+                       if want("large4096") and not args.no_large else (None, None))
+    readme = readme_section(pp, synth, torch, tuning, stream) if want("readme") else None
 
     # ---------------------------------------------------- Rothermel field
     field = synth.rothermel_field(seed=0, side=1024)
-    bf = torch.from_numpy(field["fuel"]).to(dev)
-    bc = [torch.from_numpy(field[k]).to(dev) for k in FIELDS]
-    bo = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
-    for _ in range(3):
-        pp.burn_device(bf, *bc, *bo, stream=stream)
-    torch.cuda.synchronize()
-    reps = 50
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        pp.burn_device(bf, *bc, *bo, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    burn_ms = e0.elapsed_time(e1) / reps
-    burn_mcells = n / (burn_ms * 1e-3) / 1e6
-    burn_gbs = 49.0 * n / (burn_ms * 1e-3) / 1e9
-    log(f"rothermel: {burn_ms * 1e3:.1f} us per 1024^2 pass, {burn_mcells:.0f} Mcells/s, {burn_gbs:.0f} GB/s")
+    rothermel = None
+    if want("rothermel"):
+        bf = torch.from_numpy(field["fuel"]).to(dev)
+        bc = [torch.from_numpy(field[k]).to(dev) for k in FIELDS]
+        bo = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+        for _ in range(3):
+            pp.burn_device(bf, *bc, *bo, stream=stream)
+        torch.cuda.synchronize()
+        reps = 50
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            pp.burn_device(bf, *bc, *bo, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        burn_ms = e0.elapsed_time(e1) / reps
+        burn_mcells = n / (burn_ms * 1e-3) / 1e6
+        burn_gbs = 49.0 * n / (burn_ms * 1e-3) / 1e9
+        log(f"rothermel: {burn_ms * 1e3:.1f} us per 1024^2 pass, {burn_mcells:.0f} Mcells/s, {burn_gbs:.0f} GB/s")
+        rothermel = {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,
+                     "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS, "workload": "rothermel_field_1024"}
 
     # ------------------------------------------------ moisture pre-pass
-    mo_outs = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
-    mo_slope, mo_aspect = bc[FIELDS.index("slope")], bc[FIELDS.index("aspect")]
-    hours = np.arange(24)
-    mo_w = pp.day_weather(12 + 10 * np.sin(hours / 24 * 2 * np.pi), 60 - 25 * np.sin(hours / 24 * 2 * np.pi),
-                          30.0, [0.0, 0.0, 3.0, 0.0, 12.0, 0.0], 7)
-    for _ in range(3):
-        pp.moisture_device(bf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
-    e0.record(stream)
-    for _ in range(reps):
-        pp.moisture_device(bf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    mo_ms = e0.elapsed_time(e1) / reps
-    mo_gbs = 29.0 * n / (mo_ms * 1e-3) / 1e9
-    log(f"moisture pre-pass: {mo_ms * 1e3:.1f} us per 1024^2 hour, {n / (mo_ms * 1e-3) / 1e6:.0f} Mcells/s, "
-        f"{mo_gbs:.0f} GB/s")
+    moisture = None
+    if want("moisture"):
+        mf = torch.from_numpy(field["fuel"]).to(dev)
+        mo_slope = torch.from_numpy(field["slope"]).to(dev)
+        mo_aspect = torch.from_numpy(field["aspect"]).to(dev)
+        mo_outs = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(5)]
+        hours = np.arange(24)
+        mo_w = pp.day_weather(12 + 10 * np.sin(hours / 24 * 2 * np.pi), 60 - 25 * np.sin(hours / 24 * 2 * np.pi),
+                              30.0, [0.0, 0.0, 3.0, 0.0, 12.0, 0.0], 7)
+        for _ in range(3):
+            pp.moisture_device(mf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
+        reps = 50
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            pp.moisture_device(mf, mo_slope, mo_aspect, mo_w, 14, *mo_outs, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        mo_ms = e0.elapsed_time(e1) / reps
+        mo_gbs = 29.0 * n / (mo_ms * 1e-3) / 1e9
+        log(f"moisture pre-pass: {mo_ms * 1e3:.1f} us per 1024^2 hour, {n / (mo_ms * 1e-3) / 1e6:.0f} Mcells/s, "
+            f"{mo_gbs:.0f} GB/s")
+        moisture = {"value": n / (mo_ms * 1e-3) / 1e6, "unit": "Mcells/s", "us_per_pass": mo_ms * 1e3,
+                    "achieved_gbs": mo_gbs, "frac": mo_gbs / HBM_PEAK_GBS,
+                    "workload": "moisture pre-pass, 1024x1024 i.i.d. fuel/slope/aspect, July day, 14:00"}
 
     if args.copy_probe_mib:
         src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
@@ -503,33 +685,38 @@ def main():
 
     # -------------------------------------------------------- CPU baseline
     cpu = None
-    if not args.no_cpu_baseline:
-        import oracle
+    if not args.no_cpu_baseline and want("cpu"):
+        c2 = None
+        if large is not None:
+            c2 = (large_sc[0], {"time_host": large_sc[1], "sweeps": large["sweeps"]})
+        cpu = cpu_section(args, sc, field, c2, wl2048)
 
-        t0 = time.perf_counter()
-        cpu_fire = oracle.burn_f32(sc["terrain"], nthreads=1)
-        og = oracle.grid(W, H, sc["dx"], sc["dy"], sc["max_time"])
-        ref = oracle.propagate(og, *cpu_fire, sc["time"], sc["refs_x"], sc["refs_y"], nthreads=1)
-        cpu_s = time.perf_counter() - t0
-        cores = min(16, len(os.sched_getaffinity(0)))
-        t1 = time.perf_counter()
-        oracle.burn_f32(field, nthreads=cores)
-        cpu_burn_s = time.perf_counter() - t1
-        cpu = {"value": 1.0 / cpu_s, "unit": "scenarios/s", "cores": 1, "kind": "port",
-               "sample": "one full config-1 scenario (burn + serial exact-Jacobi propagation to "
-                         f"convergence, {ref['sweeps']} sweeps) on 1 thread, {cpu_s:.2f} s",
-               "rothermel_mcells_s": n / cpu_burn_s / 1e6, "rothermel_cores": cores}
-        log(f"cpu baseline: {cpu_s:.2f} s per scenario (1 thread); burn {n / cpu_burn_s / 1e6:.1f} "
-            f"Mcells/s on {cores} threads")
+    if args.only is not None:
+        sect = {"config1": results.get("config1"), "plain": results.get("plain"),
+                "ensemble": {k: v for k, v in (results.get("ensemble") or {}).items() if k != "stats"},
+                "serving": serving, "ensemble2048": ens2048, "large4096": large, "readme": readme,
+                "rothermel": rothermel, "moisture": moisture, "cpu": cpu}[args.only]
+        print(json.dumps({"section": args.only, "result": sect}, default=float), flush=True)
+        return
 
+    ens = results["ensemble"]
+    es = ens["stats"]
+    c1 = results["config1"]
+    plain = results["plain"]
+    cpu_line = None
+    if cpu is not None:
+        cpu_line = dict(cpu["config1"])
+        cpu_line.update({"rothermel_mcells_s": cpu["rothermel"]["value"], "rothermel_cores": cpu["rothermel"]["cores"],
+                         "host": cpu["host"], "large_4096": cpu.get("large_4096"),
+                         "ensemble_2048": cpu.get("ensemble_2048")})
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": ens["value"],
         "unit": "scenarios/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": ens["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -539,40 +726,27 @@ def main():
                    "cells_per_scenario": n, "cell_m": 5.0, "max_time_s": 36000.0,
                    "form": "batched (one shared persistent launch per ensemble)",
                    "schedule": "one launch per sweep" if args.mode == 1 else "persistent",
-                   "tuning": btuning},
-        "latency_ms": {"ensemble_p50": float(np.percentile(ens_ms, 50)),
-                       "ensemble_p99": float(np.percentile(ens_ms, 99)),
-                       "plain_p50": float(np.percentile(plain_lat, 50)),
-                       "plain_p99": float(np.percentile(plain_lat, 99))},
-        "plain": {"value": plain_sps, "unit": "scenarios/s", "scenarios": n_plain,
-                  "speedup_batched": value / plain_sps, "parity_first_scenarios": bool(parity),
-                  "config1_ms": float(np.mean(c1_lat)), "config1_p50_ms": float(np.percentile(c1_lat, 50)),
-                  "config1_p99_ms": float(np.percentile(c1_lat, 99))},
-        "autotune": tune_1024,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "traffic_source": TRAFFIC_PROFILE,
-                     "kernel": "propagate_sweep" if args.mode == 1 else "propagate_persistent",
-                     "kernel_ms": ens_relax_ms, "algorithmic_bytes": alg_bytes},
-        "rothermel": {"value": burn_mcells, "unit": "Mcells/s", "us_per_pass": burn_ms * 1e3,
-                      "achieved_gbs": burn_gbs, "frac": burn_gbs / HBM_PEAK_GBS,
-                      "workload": "rothermel_field_1024"},
+                   "tuning": ens["tuning"]},
+        "latency_ms": {"ensemble_p50": float(np.percentile(ens["ens_ms"], 50)),
+                       "ensemble_p99": float(np.percentile(ens["ens_ms"], 99)),
+                       "plain_p50": float(np.percentile(plain["lat"], 50)),
+                       "plain_p99": float(np.percentile(plain["lat"], 99))},
+        "plain": {"value": plain["value"], "unit": "scenarios/s", "scenarios": plain["scenarios"],
+                  "speedup_batched": ens["value"] / plain["value"], "parity_first_scenarios": bool(ens["parity"]),
+                  "config1_ms": c1["ms"], "config1_p50_ms": c1["p50_ms"], "config1_p99_ms": c1["p99_ms"]},
+        "autotune": ens["autotune"],
+        "roofline": ens["roofline"],
+        "rothermel": rothermel,
         "serving": serving,
         "ensemble_2048": ens2048,
         "large_4096": large,
-        "moisture": {"value": n / (mo_ms * 1e-3) / 1e6, "unit": "Mcells/s", "us_per_pass": mo_ms * 1e3,
-                     "achieved_gbs": mo_gbs, "frac": mo_gbs / HBM_PEAK_GBS,
-                     "workload": "moisture pre-pass, 1024x1024 i.i.d. fuel/slope/aspect, July day, 14:00"},
-        "cpu_baseline": cpu,
+        "readme_scenario": readme,
+        "moisture": moisture,
+        "cpu_baseline": cpu_line,
         "propagation": {"ensemble": {k: es[k] for k in ("iterations", "relaxations", "improvements", "dda_steps",
-                                                       "tiles_visited", "tile_marks", "idle_tiles", "relax_ms", "elapsed_ms",
-                                                       "workgroups", "work_ms", "wait_ms", "crit_ms")},
-                        "config1": {"sweeps": st1["iterations"], "relaxations": st1["relaxations"],
-                                    "improvements": st1["improvements"], "dda_steps": st1["dda_steps"],
-                                    "tiles_visited": st1["tiles_visited"], "burned_cells": burned,
-                                    "relax_ms": relax_ms, "crit_ms": st1["crit_ms"],
-                                    "phase_ms": st1["phase_ms"], "barrier_ms": st1["barrier_ms"],
-                                    "lds_bitset": st1["lds_bitset"], "workgroups": st1["workgroups"]}},
+                                                       "tiles_visited", "tile_marks", "idle_tiles", "relax_ms",
+                                                       "elapsed_ms", "workgroups", "work_ms", "wait_ms", "crit_ms")},
+                        "config1": c1["stats"]},
     }
     print(json.dumps(out), flush=True)
 
