@@ -96,6 +96,16 @@ const char *pg_last_error(void);
 /* Version string of the library build. */
 const char *pg_version(void);
 
+/* Releases every device resource the library holds outside batching
+ * contexts: the propagation workspaces, the cached launch graphs with their
+ * events and pinned control blocks, and the device fuel catalog, on every
+ * device used so far; a later call recreates what it needs.  Call it with
+ * no call of the library in flight (contexts are released by
+ * pg_ctx_destroy).  It also runs by itself at process exit, before the HIP
+ * runtime's own exit handlers.  No counterpart in the reference (its cust
+ * objects are dropped by Rust). */
+pg_status pg_shutdown(void);
+
 /* ---------------------------------------------------------------- Stage F */
 
 /* Host-only diagnostic (no GPU needed): writes the f32 catalog constants of
@@ -233,6 +243,20 @@ pg_status pg_rasterize_times(const GeoReference *geo, const FFITimeFeature *feat
  * replaces rasterize_fuels (crates/propag/src/propag.rs:929-978). */
 pg_status pg_rasterize_fuels(const GeoReference *geo, const FFIFuelFeature *features,
                              size_t n, uint8_t *fuel, uint32_t *burned);
+
+/* Transforms n points in place from one proj4 CRS to another: geographic
+ * (longitude, latitude in degrees) and transverse Mercator (+proj=utm,
+ * +proj=tmerc) on the common ellipsoids, x = east, y = north.  ok[i] is 0
+ * (and the point unchanged) where a point lies outside the target
+ * projection's domain.  Two CRSs on different datums are related without a
+ * shift (PROJ's default when none is defined); a pair that needs a real
+ * shift, or a CRS outside that set, returns PG_ERR_INVALID with the reason.
+ * Replaces, for points, the OGR transformation the reference applies to
+ * every ignition and fuel geometry before rasterising it
+ * (Geometry::transform_to_inplace, crates/propag/src/propag.rs:854-857,
+ * 964-969).  Host code; needs no GPU. */
+pg_status pg_transform_points(const char *src_proj, const char *dst_proj, size_t n, double *x,
+                              double *y, uint8_t *ok);
 
 /* Initial references of propagate() (crates/propag/src/propag.rs:433-449):
  * each cell whose time is not FLT_MAX points at itself, every other cell

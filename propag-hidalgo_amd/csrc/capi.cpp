@@ -5,6 +5,7 @@
 // reference's bool + strncpy into a caller buffer
 // (crates/propag/src/propag.rs:274-309), which can leave the buffer
 // unterminated.
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -42,9 +43,33 @@ pg_status hip_error(hipError_t e, const char *what) {
   return set_error(e == hipErrorOutOfMemory ? PG_ERR_OUT_OF_MEMORY : PG_ERR_HIP, msg);
 }
 
+void release_all() {
+  release_launch_resources();
+  std::lock_guard<std::mutex> lk(g_dev_mutex);
+  int current = 0;
+  const bool have = hipGetDevice(&current) == hipSuccess;
+  for (size_t d = 0; d < g_devices.size(); ++d) {
+    DeviceInfo *info = g_devices[d];
+    if (!info) continue;
+    if (hipSetDevice(static_cast<int>(d)) == hipSuccess && info->catalog) (void)hipFree(info->catalog);
+    delete info;
+    g_devices[d] = nullptr;
+  }
+  if (have) (void)hipSetDevice(current);
+  (void)hipGetLastError();
+}
+
+// Registered once, when the library first sets up a device: by then the HIP
+// runtime (and any tool wrapping it) is loaded and has registered its own
+// exit handlers, so this one runs before them and the library's device
+// memory, graphs and pinned blocks are released while the runtime is whole.
+void release_at_exit() { release_all(); }
+
 pg_status device_info(DeviceInfo **out) {
   int dev = 0;
   PG_HIP(hipGetDevice(&dev));
+  static const bool registered = std::atexit(release_at_exit) == 0;
+  (void)registered;
   std::lock_guard<std::mutex> lk(g_dev_mutex);
   if (g_devices.size() <= static_cast<size_t>(dev)) g_devices.resize(dev + 1, nullptr);
   if (!g_devices[dev]) {
@@ -77,7 +102,12 @@ extern "C" {
 
 const char *pg_last_error(void) { return t_last_error.c_str(); }
 
-const char *pg_version(void) { return "propag-hip 0.1.0 (gfx950)"; }
+const char *pg_version(void) { return "propag-hip 0.2.0 (gfx950)"; }
+
+pg_status pg_shutdown(void) {
+  release_all();
+  return PG_OK;
+}
 
 int pg_catalog_f32(int code, float out[16]) {
   if (code < 0 || code >= kNumFuels || !out) return -1;

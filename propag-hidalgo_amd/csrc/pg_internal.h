@@ -65,6 +65,10 @@ struct DeviceInfo {
 };
 pg_status device_info(DeviceInfo **out);
 
+// Frees the propagation's per-device workspaces and cached launch graphs
+// (propag.hip).
+void release_launch_resources();
+
 // Stage F launcher (burn.hip).
 pg_status launch_burn(const DeviceInfo *dev, size_t n, const uint8_t *fuel,
                       const float *d1, const float *d10, const float *d100,

@@ -13,11 +13,12 @@
 // catches panics the same way, 274-309).
 //
 // Where it differs on purpose: no MPI initialisation, so a process may run
-// any number of propagations (the reference's second call fails); the
-// ignition CRS must be the raster's own, compared as proj4 parameter sets,
-// because no reprojection library is linked; the boundary layers describe
-// this implementation's launch decomposition (8 x 8 tiles in one launch
-// over the whole raster) instead of CUDA blocks and super-grid tiles.
+// any number of propagations (the reference's second call fails); features
+// are reprojected to the raster's CRS by reproject.h instead of PROJ
+// (geographic and transverse Mercator CRSs, no datum shifts; other CRSs are
+// refused with an error); the boundary layers describe this
+// implementation's launch decomposition (8 x 8 tiles in one launch over the
+// whole raster) instead of CUDA blocks and super-grid tiles.
 
 #include <algorithm>
 #include <cmath>
@@ -28,6 +29,7 @@
 
 #include "../../include/propag_host.h"
 #include "crs.h"
+#include "reproject.h"
 
 namespace {
 
@@ -43,9 +45,17 @@ std::string last(const char *what) { return std::string(what) + ": " + pg_last_e
 // The reference's some_path: NULL or "" is no path.
 const char *some_path(const char *p) { return p && *p ? p : nullptr; }
 
-// SpatialRef::from_proj4 of the features' CRS, then transform_to_inplace to
-// the raster's: only the identity is available here.
-bool check_crs(const char *crs, const GeoReference &geo, std::string &err) {
+// SpatialRef::from_proj4 of the features' CRS and of the raster's, then
+// Geometry::transform_to_inplace of each feature (propag.rs:850-858,
+// 960-970): features whose WKB parses are rewritten into `bufs` with their
+// vertices in the raster's CRS; one with a vertex outside the target
+// projection's domain is dropped, as the reference's filter_map drops a
+// failed transformation; bytes that are not WKB are passed on for the
+// rasteriser to skip.  With the raster's own CRS the features pass
+// unchanged.
+template <class Feature>
+bool reproject(const char *crs, const GeoReference &geo, const Feature *in, size_t n, std::vector<Feature> &out,
+               std::vector<std::vector<uint8_t>> &bufs, std::string &err) {
   if (!crs || !*crs) {
     err = "invalid CRS: empty proj4 string";
     return false;
@@ -55,17 +65,32 @@ bool check_crs(const char *crs, const GeoReference &geo, std::string &err) {
     err = "invalid CRS: the GeoReference has no proj4 string";
     return false;
   }
-  if (!pg::same_crs(crs, raster)) {
-    err = std::string("features in '") + crs + "' would need reprojection to '" + raster +
-          "', which this library does not link; pass them in the raster's CRS";
-    return false;
+  pg::Transform tr;
+  if (!tr.init(crs, raster, err)) return false;
+  out.assign(in, in + n);
+  if (tr.identity()) return true;
+  bufs.assign(n, {});
+  size_t kept = 0;
+  for (size_t i = 0; i < n; ++i) {
+    Feature f = in[i];
+    if (f.geom_wkb && f.geom_wkb_len) {
+      const int st = pg::WkbTransformer(tr, f.geom_wkb, f.geom_wkb_len, bufs[i]).run();
+      if (st == 2) continue;  // outside the target projection: dropped
+      if (st == 0) f.geom_wkb = bufs[i].data();
+    }
+    out[kept++] = f;
   }
+  out.resize(kept);
   return true;
 }
 
 bool run(const FFIPropagation &p, std::string &err) {
   const GeoReference &geo = p.settings.geo_ref;
-  if (!check_crs(p.initial_ignited_elements_crs, geo, err)) return false;
+  std::vector<FFITimeFeature> features;
+  std::vector<std::vector<uint8_t>> wkbs;
+  const size_t nin = p.initial_ignited_elements ? p.initial_ignited_elements_len : 0;
+  if (!reproject(p.initial_ignited_elements_crs, geo, p.initial_ignited_elements, nin, features, wkbs, err))
+    return false;
   const char *out_path = some_path(p.output_path);
   if (!out_path) {
     err = "invalid output_path";
@@ -73,8 +98,7 @@ bool run(const FFIPropagation &p, std::string &err) {
   }
   const size_t n = static_cast<size_t>(geo.width) * geo.height;
   std::vector<float> time(n);
-  const size_t nfeat = p.initial_ignited_elements ? p.initial_ignited_elements_len : 0;
-  if (pg_rasterize_times(&geo, p.initial_ignited_elements, nfeat, time.data(), nullptr) != PG_OK) {
+  if (pg_rasterize_times(&geo, features.data(), features.size(), time.data(), nullptr) != PG_OK) {
     err = last("rasterize_times");
     return false;
   }
@@ -191,10 +215,14 @@ bool propag_rasterize_fuels(const FFIFuelFeature *fuels, uintptr_t fuels_len, co
   try {
     if (!geo_ref || !result) {
       err = "null GeoReference or result";
-    } else if (check_crs(fuels_crs, *geo_ref, err)) {
+    } else {
+      std::vector<FFIFuelFeature> feats;
+      std::vector<std::vector<uint8_t>> wkbs;
       const size_t n = fuels && fuels_len ? fuels_len : 0;
-      if (pg_rasterize_fuels(geo_ref, fuels, n, result, nullptr) == PG_OK) return true;
-      err = last("rasterize_fuels");
+      if (reproject(fuels_crs, *geo_ref, fuels, n, feats, wkbs, err)) {
+        if (pg_rasterize_fuels(geo_ref, feats.data(), feats.size(), result, nullptr) == PG_OK) return true;
+        err = last("rasterize_fuels");
+      }
     }
   } catch (const std::exception &e) {
     err = e.what();

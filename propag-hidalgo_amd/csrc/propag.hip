@@ -1749,6 +1749,33 @@ void free_graph(LaunchGraph *lg) {
   delete lg;
 }
 
+}  // namespace
+
+// Frees every per-device workspace and cached launch graph (the next call
+// recreates what it needs).  Runs with no propagation in flight.
+void release_launch_resources() {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  int current = 0;
+  const bool have = hipGetDevice(&current) == hipSuccess;
+  for (size_t d = 0; d < g_dev.size(); ++d) {
+    PerDevice *pd = g_dev[d];
+    if (!pd) continue;
+    if (hipSetDevice(static_cast<int>(d)) != hipSuccess) continue;
+    (void)hipDeviceSynchronize();
+    for (LaunchGraph *lg : pd->graphs) free_graph(lg);
+    pd->graphs.clear();
+    for (DeviceBuffer *b : {&pd->call, &pd->launch}) {
+      if (b->base) (void)hipFree(b->base);
+      b->base = nullptr;
+      b->bytes = 0;
+    }
+  }
+  if (have) (void)hipSetDevice(current);
+  (void)hipGetLastError();
+}
+
+namespace {
+
 // The cached graph for this launch shape (built on first use).
 pg_status launch_graph_for(PerDevice *pd, void (*kern)(Args), unsigned groups, unsigned block, size_t shmem,
                            const Args &a, size_t max_groups, LaunchGraph **out) {

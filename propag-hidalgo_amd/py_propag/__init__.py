@@ -17,10 +17,12 @@ Two levels:
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
 import os
 import struct
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -608,6 +610,25 @@ __all__ = [
 
 RUNNING, DONE, NOT_CONVERGED, FAILED = 0, 1, 2, 3
 
+_open_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _release_at_exit():
+    """Closes the contexts still open and releases the library's device
+    resources while the interpreter and the HIP runtime are whole (instead
+    of leaving them to interpreter finalisation and the runtime's exit)."""
+    for ctx in list(_open_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+    if _native._lib is not None:
+        try:
+            _native._lib.pg_shutdown()
+        except Exception:
+            pass
+
 
 class Context:
     """Device-resident batching context (the pg_ctx_* handle API).
@@ -625,6 +646,7 @@ class Context:
         self._lib, self._h = lib, h
         self._shapes = {}     # terrain id -> (height, width)
         self._scen = {}       # scenario id -> terrain id
+        _open_contexts.add(self)
 
     def close(self):
         if self._h:

@@ -18,7 +18,7 @@ EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propa
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
            "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture",
            "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs", "pg_write_times_geotiff",
-           "pg_write_refs_gpkg", "pg_write_tiles_gpkg", "pg_propagate_noalloc")
+           "pg_write_refs_gpkg", "pg_write_tiles_gpkg", "pg_propagate_noalloc", "pg_shutdown", "pg_transform_points")
 
 
 class GT(C.Structure):
@@ -123,6 +123,10 @@ def load(require_torch: bool = True):
     lib.pg_propagate.restype = C.c_int
     lib.pg_propagate.argtypes = [C.POINTER(SettingsC)] + [vp] * 7 + [C.POINTER(Tuning),
                                                                      C.POINTER(Stats), vp]
+    lib.pg_transform_points.restype = C.c_int
+    lib.pg_transform_points.argtypes = [C.c_char_p, C.c_char_p, sz, vp, vp, vp]
+    lib.pg_shutdown.restype = C.c_int
+    lib.pg_shutdown.argtypes = []
     lib.pg_propagate_noalloc.restype = C.c_int
     lib.pg_propagate_noalloc.argtypes = lib.pg_propagate.argtypes
     lib.pg_propagate_host.restype = C.c_int

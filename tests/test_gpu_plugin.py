@@ -88,3 +88,60 @@ def test_ffipropagation_run_matches_the_oracle(oracle_lib, tmp_path, north_up):
     ok, msg = plugin.run(settings, str(tmp_path / "t2.tif"), ign, UTM, fill)
     assert ok, msg
     assert np.array_equal(read_tiff(str(tmp_path / "t2.tif"))[1].view(np.uint32), got.view(np.uint32))
+
+
+def test_ignitions_in_another_crs_are_reprojected(oracle_lib, tmp_path):
+    """FFIPropagation_run with the ignitions in longitude/latitude (the QGIS
+    plugin passes the ignition layer's own CRS, qgsalgorithmpropag.cpp:
+    188-216,247-250) writes the same GeoTIFF as with the same ignitions in
+    the raster's UTM coordinates: the features are reprojected before they
+    are rasterised (propag.rs:854-857).  Vertices sit at least 0.1 cell from
+    pixel centres and edges, so the round trip's sub-millimetre differences
+    cannot move a burn."""
+    import ctypes as C
+
+    import py_propag as pp
+    from py_propag import _native, plugin
+
+    lib = _native.load()
+    side = 96
+    sc = synth.single_scenario(seed=5, side=side)
+    dx = float(sc["dx"])
+    x0, y0 = 440000.0, 4470000.0
+    geo = pp.GeoReference(side, side, UTM, [x0, dx, 0.0, y0, 0.0, dx])
+    settings = pp.Settings(geo, float(sc["max_time"]))
+    world = lambda px, py: (x0 + px * dx, y0 + py * dx)  # noqa: E731
+    parts = [(0.0, R.POINT, world(40.5, 41.5)),
+             (600.0, R.LINESTRING, [world(10.2, 70.7), world(30.9, 60.1)]),
+             (1200.0, R.POLYGON, [[world(70.3, 20.2), world(80.6, 20.2), world(80.6, 27.9), world(70.3, 27.9),
+                                   world(70.3, 20.2)]])]
+    wgs84 = "+proj=longlat +datum=WGS84 +no_defs"
+
+    def to_lonlat(pts):
+        pts = np.asarray(pts, float).reshape(-1, 2)
+        x, y = pts[:, 0].copy(), pts[:, 1].copy()
+        ok = np.zeros(len(pts), np.uint8)
+        assert lib.pg_transform_points(UTM.encode(), wgs84.encode(), len(pts), x.ctypes.data, y.ctypes.data,
+                                       ok.ctypes.data) == 0
+        assert ok.all()
+        return [tuple(p) for p in np.column_stack([x, y])]
+
+    ign_utm = [(t, R.wkb(kind, coords)) for t, kind, coords in parts]
+    ign_ll = [(0.0, R.wkb(R.POINT, to_lonlat(parts[0][2])[0])),
+              (600.0, R.wkb(R.LINESTRING, to_lonlat(parts[1][2]))),
+              (1200.0, R.wkb(R.POLYGON, [to_lonlat(parts[2][2][0])]))]
+    ter = sc["terrain"]
+
+    def fill(g, cols):
+        cols["fuel"][:] = ter["fuel"]
+        for k in plugin.TERRAIN_COLUMNS:
+            cols[k][:] = ter[k]
+        return True
+
+    out = []
+    for ign, crs, name in ((ign_utm, UTM, "utm.tif"), (ign_ll, wgs84, "ll.tif")):
+        ok, msg = plugin.run(settings, str(tmp_path / name), ign, crs, fill)
+        assert ok, msg
+        out.append(read_tiff(str(tmp_path / name))[1])
+    assert (out[0] < R.FLT_MAX).sum() > 1000
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))

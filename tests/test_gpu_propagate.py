@@ -293,3 +293,31 @@ def test_wide_rasters_uncached_path(oracle_lib, W, H, lds, mode):
     assert gpu["stats"]["iterations"] == ref["sweeps"]
     if mode == 0:
         assert gpu["stats"]["lds_bitset"] == lds
+
+
+def test_shutdown_releases_and_the_next_call_recreates(oracle_lib):
+    """pg_shutdown frees the workspaces, cached launch graphs (built by a
+    mode-2 batched launch first) and the device catalog; the calls after it
+    rebuild them and give the same results."""
+    import py_propag as pp
+    from py_propag import _native
+
+    sc = synth.single_scenario(seed=6, side=128)
+    fire = _fire(sc)
+    before = gpu_propagate(sc, fire)
+    burn_before = pp.burn(sc["terrain"])
+    w = 96
+    land = synth.landscape(3, w, 2, 8)
+    g = pp.GeoReference(w, w, "+proj=utm +zone=30", [0.0, 5.0, 0.0, 0.0, 0.0, 5.0])
+    wx = {"d1hr": 0.06, "d10hr": 0.07, "d100hr": 0.08, "herb": 0.9, "wood": 1.0, "wind_speed": 5.0,
+          "wind_azimuth": 1.0}
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(g, land["fuel"], land["slope"], land["aspect"])
+        ctx.admit(tid, wx, [(40, 40, 0.0)], 3600.0)
+        ctx.run(tuning={"mode": 2})
+    assert _native.load().pg_shutdown() == 0
+    after = gpu_propagate(sc, fire)
+    for k in ("time", "refs_x", "refs_y", "boundaries"):
+        assert np.array_equal(before[k], after[k]), k
+    for a, b in zip(burn_before, pp.burn(sc["terrain"])):
+        assert np.array_equal(a, b)

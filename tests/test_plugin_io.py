@@ -307,9 +307,13 @@ def test_propag_rasterize_fuels(plug):
     # the same CRS written with its parameters in another order is the same CRS
     shuffled = " ".join(reversed(UTM.split()))
     assert lib.propag_rasterize_fuels(arr, len(feats), shuffled.encode(), C.byref(gc), out.ctypes.data, err, 256)
-    assert not lib.propag_rasterize_fuels(arr, len(feats), b"+proj=longlat +datum=WGS84", C.byref(gc),
+    # CRSs outside the transformable set, and pairs that need a datum shift, are refused
+    assert not lib.propag_rasterize_fuels(arr, len(feats), b"+proj=merc +datum=WGS84", C.byref(gc),
                                           out.ctypes.data, err, 256)
-    assert b"reprojection" in err.value
+    assert b"not supported" in err.value
+    assert not lib.propag_rasterize_fuels(arr, len(feats), b"+proj=longlat +ellps=airy +towgs84=446.4,-125.2,542.1",
+                                          C.byref(gc), out.ctypes.data, err, 256)
+    assert b"datum shift" in err.value
     small = C.create_string_buffer(b"\xff" * 8)
     assert not lib.propag_rasterize_fuels(arr, len(feats), b"", C.byref(gc), out.ctypes.data, small, 8)
     assert small.raw[7] == 0 and len(small.value) == 7                  # truncated and terminated
@@ -336,8 +340,8 @@ def test_run_reports_errors_before_the_gpu(plug, tmp_path):
     assert all(cols[k] == (np.float32, W * H, 0) for k in plug.TERRAIN_COLUMNS)
     ok, msg = plug.run(s, "", ign, UTM, fill)
     assert not ok and msg == "invalid output_path"
-    ok, msg = plug.run(s, str(tmp_path / "t.tif"), ign, "+proj=longlat +datum=WGS84", fill)
-    assert not ok and "reprojection" in msg
+    ok, msg = plug.run(s, str(tmp_path / "t.tif"), ign, "+proj=lcc +lat_1=40 +datum=WGS84", fill)
+    assert not ok and "not supported" in msg
     ok, msg = plug.run(s, str(tmp_path / "t.tif"), ign, None, fill)
     assert not ok and "CRS" in msg
     assert len(calls) == 1 and not os.path.exists(tmp_path / "t.tif")
