@@ -46,26 +46,33 @@ FIELDS = ("d1hr", "d10hr", "d100hr", "herb", "wood", "wind_speed", "wind_azimuth
           "aspect")
 
 
-TRAFFIC_PROFILE = "profiles/propagate_batched_ensemble_1024_hbm_traffic.md"
+TRAFFIC_PROFILE = "profiles/propagate_hbm_traffic.md"
 
 
-def roofline(relaxations, improvements, kernel_ms):
+def roofline(relaxations, improvements, kernel_ms, workload=None):
     """HBM roofline of a relaxation kernel: algorithmic bytes (24 B per cell
-    relaxation + 8 B per committed improvement) over its HIP-event time."""
+    relaxation + 8 B per committed improvement) over its HIP-event time, with
+    the HBM bytes per launch that rocprofv3 measured for `workload`."""
     alg = 24.0 * relaxations + 8.0 * improvements
     gbs = alg / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "algorithmic_bytes": alg, "kernel_ms": kernel_ms}
+    out = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "algorithmic_bytes": alg, "kernel_ms": kernel_ms, "kernel": "propagate_persistent"}
+    if workload is not None:
+        t = measured_traffic(workload)
+        out.update({"traffic": t, "traffic_source": f"{TRAFFIC_PROFILE} ({workload})",
+                    "traffic_over_algorithmic": t / alg if t and alg else None})
+    return out
 
 
-def measured_traffic():
-    """HBM bytes per launch of the batched kernel, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters cannot be read from
-    inside this process); None if it is absent."""
+def measured_traffic(workload):
+    """HBM bytes per launch of the relaxation kernel on `workload`, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters cannot be
+    read from inside this process); None if it is absent."""
+    key = f"traffic_bytes_per_launch {workload}:"
     try:
         with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return int(line.split(":")[1])
     except OSError:
         pass
@@ -159,9 +166,12 @@ def large_single(args, pp, synth, torch, tuning, stream):
     out = {"workload": "large_scenario_4096 (16 points + 200-cell line, 12 h)", "ms": float(np.mean(ms)),
            "ms_min": float(np.min(ms)), "relax_ms": float(np.mean([r[1]["relax_ms"] for r in runs])),
            "sweeps": st["iterations"], "relaxations": st["relaxations"], "improvements": st["improvements"],
-           "tiles_visited": st["tiles_visited"], "dda_steps": st["dda_steps"], "burned_cells": burned,
+           "tiles_visited": st["tiles_visited"], "tile_marks": st["tile_marks"], "idle_tiles": st["idle_tiles"],
+           "dda_steps": st["dda_steps"], "burned_cells": burned,
            "repeatable": bool(same), "ignitions": len(sc["ignitions"])}
-    out["roofline"] = roofline(out["relaxations"], out["improvements"], out["relax_ms"])
+    if tuning.get("profile"):
+        out.update({k: st[k] for k in ("crit_ms", "phase_ms", "phase_total_ms", "barrier_ms", "idle_far_tiles")})
+    out["roofline"] = roofline(out["relaxations"], out["improvements"], out["relax_ms"], "large_4096")
     log(f"large 4096^2: {out['ms']:.2f} ms per scenario (relax {out['relax_ms']:.2f} ms), {out['sweeps']} sweeps, "
         f"{burned} burned, {out['tiles_visited']} tile visits; repeatable {same}")
     return out, (sc, t.cpu().numpy())
@@ -226,10 +236,13 @@ def ensemble_2048_section(args, pp, synth, torch, tuning, stream):
                        "log-uniform 3600-43200 s, 1-4 ignitions, uniform wind U(2,12) m/s",
            "value": value, "unit": "scenarios/s", "ms_per_ensemble": float(np.mean(ms)),
            "relax_ms": stats["relax_ms"], "sweeps": stats["iterations"], "tiles_visited": stats["tiles_visited"],
+           "tile_marks": stats["tile_marks"], "idle_tiles": stats["idle_tiles"],
            "relaxations": stats["relaxations"], "improvements": stats["improvements"], "dda_steps": stats["dda_steps"],
-           "roofline": roofline(stats["relaxations"], stats["improvements"], stats["relax_ms"]),
+           "roofline": roofline(stats["relaxations"], stats["improvements"], stats["relax_ms"], "ensemble_2048"),
            "plain_value": plain_sps, "plain_sample": len(plain_ms), "speedup_batched": value / plain_sps,
            "parity_first_scenarios": parity, "autotune": tune}
+    if tuning.get("profile"):
+        out.update({k: stats[k] for k in ("crit_ms", "phase_ms", "phase_total_ms", "barrier_ms", "idle_far_tiles")})
     log(f"ensemble 2048: {out['ms_per_ensemble']:.1f} ms per ensemble of {len(wl['cases'])} ({value:.1f} "
         f"scenarios/s; one per call {plain_sps:.1f}/s over {len(plain_ms) - 1} timed calls, "
         f"{value / plain_sps:.1f}x), relax kernel {stats['relax_ms']:.1f} ms, {stats['iterations']} sweeps, "
@@ -544,7 +557,8 @@ def main():
                       "tiles_visited": st1["tiles_visited"], "burned_cells": burned,
                       "relax_ms": relax_ms, "crit_ms": st1["crit_ms"],
                       "phase_ms": st1["phase_ms"], "barrier_ms": st1["barrier_ms"],
-                      "lds_bitset": st1["lds_bitset"], "workgroups": st1["workgroups"]}}
+                      "lds_bitset": st1["lds_bitset"], "workgroups": st1["workgroups"]},
+            "roofline": roofline(st1["relaxations"], st1["improvements"], relax_ms, "config1")}
 
     # ------------------------------------------ plain: ensemble scenarios
     plain_out = []
@@ -605,15 +619,16 @@ def main():
         plain_sps = results["plain"]["value"]
         log(f"batched: {np.mean(ens_ms):.1f} ms per ensemble of {args.batch} ({value:.1f} scenarios/s, "
             f"{value / plain_sps:.1f}x one per call); relax kernel {es['relax_ms']:.1f} ms, {es['iterations']} sweeps, "
-            f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle), "
+            f"{es['relaxations']} relaxations, {es['tiles_visited']} tile visits ({es['idle_tiles']} idle, "
+            f"{es['idle_far_tiles']} of them with no change within two cells), {es['tile_marks']} marks, "
             f"{es['dda_steps']} walk steps; parity with one-per-call: {parity}; phases (stage/rule/walk/tail) "
             f"{es['phase_ms']}, phase totals over waves {es['phase_total_ms']}, "
             f"barrier (start/span/gather/book) {es['barrier_ms']}; per workgroup: work {es['work_ms']:.1f} ms, "
             f"wait {es['wait_ms']:.1f} ms, slowest-workgroup work summed over sweeps {es['crit_ms']:.1f} ms")
         ens_relax_ms = float(np.mean([s["relax_ms"] for s in ens_stats]))
-        rl = roofline(es["relaxations"], es["improvements"], ens_relax_ms)
-        rl.update({"traffic": measured_traffic(), "traffic_source": TRAFFIC_PROFILE,
-                   "kernel": "propagate_sweep" if args.mode == 1 else "propagate_persistent"})
+        rl = roofline(es["relaxations"], es["improvements"], ens_relax_ms, "ensemble_1024")
+        if args.mode == 1:
+            rl["kernel"] = "propagate_sweep"
         results["ensemble"] = {"value": value, "wall": wall, "ens_ms": ens_ms, "stats": es, "parity": parity,
                                "tuning": btuning, "autotune": tune_1024, "roofline": rl}
 
@@ -733,7 +748,8 @@ def main():
                        "plain_p99": float(np.percentile(plain["lat"], 99))},
         "plain": {"value": plain["value"], "unit": "scenarios/s", "scenarios": plain["scenarios"],
                   "speedup_batched": ens["value"] / plain["value"], "parity_first_scenarios": bool(ens["parity"]),
-                  "config1_ms": c1["ms"], "config1_p50_ms": c1["p50_ms"], "config1_p99_ms": c1["p99_ms"]},
+                  "config1_ms": c1["ms"], "config1_p50_ms": c1["p50_ms"], "config1_p99_ms": c1["p99_ms"],
+                  "config1_roofline": c1["roofline"]},
         "autotune": ens["autotune"],
         "roofline": ens["roofline"],
         "rothermel": rothermel,

@@ -86,6 +86,9 @@ typedef struct pg_stats {
   double   phase_total_ms[4]; /* with tuning.profile: the phases of phase_ms
                                  summed over all waves and sweeps (wave-ms)  */
   uint64_t idle_tiles;      /* tile visits in which no cell needed the rule   */
+  uint64_t idle_far_tiles;  /* with tuning.profile: idle visits with no cell
+                               changed within two cells of the tile either
+                               (queued through a far-dependents box only)  */
 } pg_stats;
 
 /* ------------------------------------------------------------ lifecycle */

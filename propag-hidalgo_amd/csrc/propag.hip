@@ -477,11 +477,14 @@ struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
 static_assert(sizeof(TileLds) % 8 == 0, "TileLds must keep 8-byte alignment");
 
 
-// Per-workgroup profiling line of the grid barrier, 128 bytes each.
+// Per-workgroup profiling line of the grid barrier, 128 bytes each, one
+// half per parity of the barrier's target: a workgroup that has passed the
+// barrier may write its next sweep's line while workgroup 0 still reads
+// this sweep's, but not the one after (that needs workgroup 0's arrival).
 struct alignas(128) Slot {
-  unsigned long long ph[4];  // profiling: slowest wave's phase ticks
-  unsigned long long t_seen, t_arrive;  // profiling: sweep start / arrival
-  unsigned pad[20];
+  unsigned long long ph[2][4];              // profiling: slowest wave's phase ticks
+  unsigned long long t_seen[2], t_arrive[2];  // profiling: sweep start / arrival
+  unsigned pad[8];
 };
 static_assert(sizeof(Slot) == 128, "one slot per 128-byte line");
 
@@ -499,6 +502,7 @@ constexpr int kProfCols = 8;
 
 struct Counters {
   uint32_t relax, improve, steps, tiles, marks, idle;
+  uint32_t idle_far;                // profiling: idle visits with no change within two cells
   uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
   uint64_t crit_ticks;              // workgroup 0: slowest workgroup's work per sweep, summed
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
@@ -674,44 +678,107 @@ __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t s
   if (slot < q.cap) list_of(q, k1, x)[slot] = (static_cast<unsigned long long>(si) << 32) | low;
 }
 
-// The tiles a change of tile (tx,ty) in sweep k queues for sweep k + 1: its
-// 3x3 tile neighbourhood (lanes 0..8 of round 0) and the box of its far
-// dependents grown by one tile (bx: x0, y0, x1, y1 in lanes 0..3), box tile
-// q in lane 9 + q of round 0 and lane (q - 55) % 64 of round 1 + (q - 55) / 64.
+// The far dependents a change of a tile's far-reference cell in sweep k
+// queues for sweep k + 1: the tiles that the tile's dependents box (the
+// bounding box, in cells, of every cell that took a far reference from the
+// tile; bx: x0, y0, x1, y1 in lanes 0..3) covers once grown by one cell, as
+// the dependents' neighbours take candidates through them.
 struct NextTiles {
-  int X0, Y0, w, n;  // grown box (n = 0: none)
+  int X0, Y0, w, n;  // tile range (n = 0: none)
 };
 
 __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
   NextTiles nt{0, 0, 1, 0};
   const uint32_t x0 = __shfl(bx, 0), y0 = __shfl(bx, 1), x1 = __shfl(bx, 2), y1 = __shfl(bx, 3);
   if (x0 <= x1 && y0 <= y1) {  // an empty box is x0 = y0 = 0xFFFFFFFF, x1 = y1 = 0
-    nt.X0 = max(static_cast<int>(x0) - 1, 0);
-    nt.Y0 = max(static_cast<int>(y0) - 1, 0);
-    const int X1 = min(static_cast<int>(x1) + 1, sc.ntx - 1), Y1 = min(static_cast<int>(y1) + 1, sc.nty - 1);
+    nt.X0 = max(static_cast<int>(x0) - 1, 0) / kTile;
+    nt.Y0 = max(static_cast<int>(y0) - 1, 0) / kTile;
+    const int X1 = min(static_cast<int>(x1) + 1, sc.g.W - 1) / kTile;
+    const int Y1 = min(static_cast<int>(y1) + 1, sc.g.H - 1) / kTile;
     nt.w = X1 - nt.X0 + 1;
     nt.n = nt.w * (Y1 - nt.Y0 + 1);
   }
   return nt;
 }
 
-// imp: ballot of the tile's cells that changed (lane l = cell (l % 8, l / 8)).
-// A change of cell r in sweep k makes sweep k + 1 evaluate r's 3x3
-// neighbours and the 3x3 neighbours of every cell that references r; those
-// referencing it from afar are reached through the dependents box, the rest
-// lie within two cells of r.  So a neighbour tile is queued only if a
-// changed cell lies within two cells of the shared edge (or corner); the
-// tile itself is queued whenever it changed.
-__device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k, int tx, int ty,
-                                         const NextTiles &nt, unsigned long long imp, int lane) {
-  if (lane < 9) {
-    const int dx = lane % 3 - 1, dy = lane / 3 - 1;
-    constexpr unsigned long long kCol01 = 0x0303030303030303ull, kRow01 = 0xFFFFull;
+// The 3x3 tile neighbourhood a changed tile queues for sweep k + 1.  A
+// change of cell r in sweep k makes sweep k + 1 evaluate the cells next to
+// r and the cells next to each cell c that references r (c's neighbours
+// take a candidate through c's reference).  Those c two or more cells from
+// r are reached through the dependents box; the others are r's neighbours,
+// in the tile or its one-cell ring.  So with F = the changed cells and the
+// unchanged cells of the tile and ring whose reference is a changed
+// neighbour, a neighbour tile is queued exactly when some cell of F lies
+// next to it (tile-local x <= 0 for the tile on the left, >= kTile - 1 for
+// the one on the right, the same for y).  Returns the 9-bit set (bit
+// (dy + 1) * 3 + dx + 1), uniform over the wave; called by all lanes with
+// `imp` the ballot of the tile's changed cells (lane l = cell (l % 8, l / 8)).
+template <bool kCache>
+__device__ inline uint32_t next_neighbours(const TileLds &s, int ox, int oy, int lx, int ly, bool improved,
+                                           unsigned long long imp, int lane) {
+  // A cell at tile-local (cx, cy) whose reference is a changed neighbour in
+  // the tile.
+  auto near_dep = [&](int cx, int cy, uint32_t ref) {
+    const int rx = ref_x<kCache>(ref) - ox, ry = ref_y<kCache>(ref) - oy;
+    const bool next = abs(rx - cx) <= 1 && abs(ry - cy) <= 1 && (rx != cx || ry != cy);
+    return next && rx >= 0 && ry >= 0 && rx < kTile && ry < kTile && ((imp >> (rx + ry * kTile)) & 1ull);
+  };
+  // Bits of the neighbour tiles next to tile-local (cx, cy).
+  auto sides = [](int cx, int cy) {
+    const uint32_t xs = 2u | (cx <= 0 ? 1u : 0u) | (cx >= kTile - 1 ? 4u : 0u);
+    const uint32_t ys = 2u | (cy <= 0 ? 1u : 0u) | (cy >= kTile - 1 ? 4u : 0u);
+    return xs * (ys & 1u) | ((xs * ((ys >> 1) & 1u)) << 3) | ((xs * ((ys >> 2) & 1u)) << 6);
+  };
+  uint32_t m = 0;
+  const int cx = lx - 1, cy = ly - 1;
+  if (improved || near_dep(cx, cy, s.ref[lx + ly * kHalo])) m = sides(cx, cy);
+  // ring cell of this lane: top and bottom rows (lanes 0..2*kHalo-1), then
+  // the left and right columns
+  if (lane < 4 * kTile + 4) {
+    int rx, ry;
+    if (lane < 2 * kHalo) {
+      rx = lane % kHalo - 1;
+      ry = lane < kHalo ? -1 : kTile;
+    } else {
+      const int l = lane - 2 * kHalo;
+      rx = l < kTile ? -1 : kTile;
+      ry = l % kTile;
+    }
+    if (near_dep(rx, ry, s.ref[(rx + 1) + (ry + 1) * kHalo])) m |= sides(rx, ry);
+  }
+  uint32_t q = 0;
+#pragma unroll
+  for (int b = 0; b < 9; ++b) q |= (__ballot((m >> b) & 1u) != 0ull) ? (1u << b) : 0u;
+  return q;
+}
+
+// The same set widened to what the tile's ballot alone shows: a neighbour
+// tile is queued if a changed cell lies within two cells of it (a superset
+// of next_neighbours, without its reads and ballots: the one-scenario
+// kernel with the bitset in LDS runs its sweeps as one dependent chain, where
+// those cost more than the extra visits they save).
+__device__ inline uint32_t near_band(unsigned long long imp) {
+  constexpr unsigned long long kCol01 = 0x0303030303030303ull, kRow01 = 0xFFFFull;
+  uint32_t q = 0;
+#pragma unroll
+  for (int b = 0; b < 9; ++b) {
+    const int dx = b % 3 - 1, dy = b / 3 - 1;
     const unsigned long long selx = dx < 0 ? kCol01 : (dx > 0 ? kCol01 << (kTile - 2) : ~0ull);
     const unsigned long long sely = dy < 0 ? kRow01 : (dy > 0 ? kRow01 << (kTile * (kTile - 2)) : ~0ull);
+    q |= (imp & selx & sely) != 0ull ? 1u << b : 0u;
+  }
+  return q;
+}
+
+// Issues the marks of the queued neighbour tiles `nb` (lanes 0..8) and of
+// the box of far dependents (box tile q in lane 9 + q of round 0 and lane
+// (q - 55) % 64 of round 1 + (q - 55) / 64).
+__device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k, int tx, int ty,
+                                         const NextTiles &nt, uint32_t nb, int lane) {
+  if (lane < 9) {
+    const int dx = lane % 3 - 1, dy = lane / 3 - 1;
     const int X = tx + dx, Y = ty + dy;
-    return mark_issue(q, sc, si, k + 1, (imp & selx & sely) != 0ull && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty,
-                      X, Y);
+    return mark_issue(q, sc, si, k + 1, ((nb >> lane) & 1u) && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y);
   }
   const int b = lane - 9;
   return mark_issue(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
@@ -988,6 +1055,28 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   const bool changed = imp != 0ull;
   const bool idle = __ballot(need) == 0ull;  // (a ballot of every lane)
   if (lane == 0 && idle) ++cnt.idle;
+  if (a.prof && idle) {
+    // Diagnostics: nothing in an idle tile's halo changed; if nothing two
+    // cells from the tile changed either, only a far-dependents box queued
+    // it.  Lanes 0..43 read the ring of cells two cells from the tile.
+    bool near = false;
+    if (lane < 4 * kTile + 12) {
+      int rx, ry;
+      if (lane < kTile + 4) {
+        rx = lane - 2, ry = -2;
+      } else if (lane < 2 * kTile + 8) {
+        rx = lane - (kTile + 4) - 2, ry = kTile + 1;
+      } else if (lane < 3 * kTile + 10) {
+        rx = -2, ry = lane - (2 * kTile + 8) - 1;
+      } else {
+        rx = kTile + 1, ry = lane - (3 * kTile + 10) - 1;
+      }
+      const int rgx = ox + rx, rgy = oy + ry;
+      if (rgx >= 0 && rgy >= 0 && rgx < g.W && rgy < g.H)
+        near = ld_i32(&stamp[static_cast<size_t>(rgx) + static_cast<size_t>(rgy) * g.W]) >= k - 1;
+    }
+    if (__ballot(near) == 0ull && lane == 0) ++cnt.idle_far;
+  }
   NextTiles nt{0, 0, 1, 0};
   Mark mk0{0u, 0u, 0u, false};
   if (changed) {
@@ -996,7 +1085,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const unsigned long long refs =
         static_cast<unsigned long long>(__shfl(bx, 4)) | (static_cast<unsigned long long>(__shfl(bx, 5)) << 32);
     if (imp & refs) nt = next_tiles(sc, bx);
-    mk0 = next_round0_issue(a.q, sc, si, k, tx, ty, nt, imp, lane);
+    const uint32_t nb = kLdsBits ? near_band(imp) : next_neighbours<kCache>(s, ox, oy, lx, ly, improved, imp, lane);
+    mk0 = next_round0_issue(a.q, sc, si, k, tx, ty, nt, nb, lane);
   }
   if (need) {
     const int me = lx + ly * kHalo;
@@ -1062,7 +1152,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       const int l = __builtin_ctzll(m);
       const uint32_t rt0 = __shfl(rtile, l);
       const bool grp = far_dep && rtile == rt0;
-      m &= ~__ballot(grp);
+      const unsigned long long gm = __ballot(grp);
+      m &= ~gm;
       uint32_t lo = (grp && rbit < 32u) ? 1u << rbit : 0u, hi = (grp && rbit >= 32u) ? 1u << (rbit - 32u) : 0u;
       for (int off = 32; off > 0; off >>= 1) {
         lo |= __shfl_xor(lo, off);
@@ -1072,11 +1163,16 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[rt0]);
         if (lo) or_relaxed(r, lo);
         if (hi) or_relaxed(r + 1, hi);
+        // the group's cells' bounding box, from the ballot (lane = x + 8 y)
+        unsigned long long cols = gm | (gm >> 32);
+        cols |= cols >> 16;
+        cols |= cols >> 8;
+        const uint32_t c8 = static_cast<uint32_t>(cols & 0xFFull);
         uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt0]);
-        min_relaxed(b + 0, static_cast<uint32_t>(tx));
-        min_relaxed(b + 1, static_cast<uint32_t>(ty));
-        max_relaxed(b + 2, static_cast<uint32_t>(tx));
-        max_relaxed(b + 3, static_cast<uint32_t>(ty));
+        min_relaxed(b + 0, static_cast<uint32_t>(ox + __builtin_ctz(c8)));
+        min_relaxed(b + 1, static_cast<uint32_t>(oy + (__builtin_ctzll(gm) >> 3)));
+        max_relaxed(b + 2, static_cast<uint32_t>(ox + 31 - __builtin_clz(c8)));
+        max_relaxed(b + 3, static_cast<uint32_t>(oy + ((63 - __builtin_clzll(gm)) >> 3)));
       }
     }
   }
@@ -1238,9 +1334,10 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
     const uint64_t work = t_arrive - sweep_t0;
     cnt.work_ticks += work;
     if (a.prof) {
-      for (int q = 0; q < 4; ++q) st_u64(&slots[blockIdx.x].ph[q], wg_ph[q]);
-      st_u64(&slots[blockIdx.x].t_seen, sweep_t0);
-      st_u64(&slots[blockIdx.x].t_arrive, t_arrive);
+      const unsigned par = target & 1u;
+      for (int q = 0; q < 4; ++q) st_u64(&slots[blockIdx.x].ph[par][q], wg_ph[q]);
+      st_u64(&slots[blockIdx.x].t_seen[par], sweep_t0);
+      st_u64(&slots[blockIdx.x].t_arrive[par], t_arrive);
       wait_stores();
     }
     const uint64_t w31 = work < 0x7fffffffull ? work : 0x7fffffffull;
@@ -1303,11 +1400,12 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
         // first sweep: the kernel start of workgroup 0 stands in for the books
         const unsigned long long rel = row == 0 ? __shfl(sweep_t0, 0) : s_release;
         unsigned long long m[7] = {0, 0, 0, 0, 0, 0, 0};  // ph[4], start, span, arrival
+        const unsigned par = target & 1u;
         for (unsigned i = lane; i < G; i += kWave) {
-          for (int q = 0; q < 4; ++q) m[q] = max(m[q], ld_u64(&slots[i].ph[q]));
-          const unsigned long long ts = ld_u64(&slots[i].t_seen), ta = ld_u64(&slots[i].t_arrive);
+          for (int q = 0; q < 4; ++q) m[q] = max(m[q], ld_u64(&slots[i].ph[par][q]));
+          const unsigned long long ts = ld_u64(&slots[i].t_seen[par]), ta = ld_u64(&slots[i].t_arrive[par]);
           m[4] = max(m[4], ts > rel ? ts - rel : 0ull);
-          m[5] = max(m[5], ta - ts);
+          m[5] = max(m[5], ta > ts ? ta - ts : 0ull);
           m[6] = max(m[6], ta);
         }
         for (int q = 0; q < 7; ++q)
@@ -1350,7 +1448,7 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
   // One 64-bit add per counter per wave.
   unsigned long long v[16] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                              cnt.wait_ticks, cnt.crit_ticks, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, 0ull, 0ull, 0ull};
+                              cnt.wait_ticks, cnt.crit_ticks, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, cnt.idle_far, 0ull, 0ull};
   // phase totals are per wave (lane 0's copy only)
   const bool lead = (threadIdx.x & 63) == 0;
   for (int q = 0; q < 4; ++q) v[8 + q] = lead ? cnt.ph_total[q] : 0ull;
@@ -1398,7 +1496,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
   }
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
     const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, s_next[k & 1], lds_off, k, s, dq, cnt);
@@ -1434,7 +1532,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   const int wave = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_changed = 0;
   __syncthreads();
-  Counters cnt{0, 0, 0, 0, 0, 0, 0, 0, 0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  Counters cnt{};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
   const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], &dq[wave], cnt);
@@ -1528,10 +1626,10 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
         or_relaxed(reinterpret_cast<uint32_t *>(&sc.isref[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]) + (rb >> 5),
                    1u << (rb & 31u));
         uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]);
-        min_relaxed(b + 0, static_cast<uint32_t>(tx));
-        min_relaxed(b + 1, static_cast<uint32_t>(ty));
-        max_relaxed(b + 2, static_cast<uint32_t>(tx));
-        max_relaxed(b + 3, static_cast<uint32_t>(ty));
+        min_relaxed(b + 0, static_cast<uint32_t>(x));
+        min_relaxed(b + 1, static_cast<uint32_t>(y));
+        max_relaxed(b + 2, static_cast<uint32_t>(x));
+        max_relaxed(b + 3, static_cast<uint32_t>(y));
       }
     }
   }
@@ -2176,6 +2274,7 @@ static pg_status finish_stats(const DeviceInfo *dev, const unsigned *book, const
     stats->tiles_visited += counters[3];
     stats->tile_marks += counters[7];
     stats->idle_tiles += counters[12];
+    stats->idle_far_tiles += counters[13];
     for (int q = 0; q < 4; ++q) stats->phase_total_ms[q] += static_cast<double>(counters[8 + q]) / clock_khz;
     stats->relax_ms += rms;
     stats->work_ms += static_cast<double>(counters[4]) / clock_khz / groups_d;

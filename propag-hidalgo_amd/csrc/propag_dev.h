@@ -41,7 +41,7 @@ struct Scen {
   int *stamp;            // last sweep in which the cell changed
   const uint32_t *bits;  // boundary bitset (row-major, the output layout)
   const unsigned long long *blk;  // the same bits in 8x8 blocks (walks)
-  uint4 *box;            // per tile: tile box (x0, y0, x1, y1) of far dependents
+  uint4 *box;            // per tile: bounding box (x0, y0, x1, y1, in cells) of its far dependents
   unsigned long long *isref;  // per tile: its cells that are some cell's far reference (8x8 bits)
   uint32_t *mark[2];     // tiles queued for a sweep of parity 0 / 1
   int ntx, nty;          // tiles (= 8x8 bit blocks) per row / column

@@ -80,7 +80,8 @@ class Stats(C.Structure):
                 ("crit_ms", C.c_double),
                 ("workgroups", C.c_uint32), ("lds_bitset", C.c_uint32),
                 ("phase_ms", C.c_double * 4), ("barrier_ms", C.c_double * 4),
-                ("tile_marks", C.c_uint64), ("phase_total_ms", C.c_double * 4), ("idle_tiles", C.c_uint64)]
+                ("tile_marks", C.c_uint64), ("phase_total_ms", C.c_double * 4), ("idle_tiles", C.c_uint64),
+                ("idle_far_tiles", C.c_uint64)]
 
     def as_dict(self):
         arrays = ("phase_ms", "barrier_ms", "phase_total_ms")
