@@ -47,7 +47,10 @@ typedef enum pg_status {
 
 /* Propagation schedule knobs.  Zero fields take the library defaults. */
 typedef struct pg_tuning {
-  uint32_t tile_w, tile_h;      /* cells per tile (8 x 8; 0 = default)         */
+  uint32_t tile_w, tile_h;      /* cells per tile: 8 x 8, 16 x 4 or 32 x 2, one
+                                   cell per lane of a wave (0 x 0 = 8 x 8).  The
+                                   handle API takes the context's shape
+                                   (pg_ctx_set_tile); 0 x 0 or that shape.  */
   uint32_t workgroups;          /* persistent workgroups (0 = fill the device) */
   uint32_t max_iterations;      /* Jacobi sweep cap (0 = 64 * (W + H) + 1024)  */
   uint32_t mode;                /* 0 = persistent kernel, 1 = one launch per sweep,
@@ -325,6 +328,13 @@ enum { PG_SCENARIO_RUNNING = 0, PG_SCENARIO_DONE = 1, PG_SCENARIO_NOT_CONVERGED 
  * context allocates for scenarios (0: 90 % of the free device memory). */
 pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **ctx);
 pg_status pg_ctx_destroy(pg_ctx *ctx);
+
+/* Sets the tile shape (8 x 8 by default; 16 x 4 or 32 x 2) of the scenarios
+ * the context admits from now on; PG_ERR_BUSY while any scenario is admitted
+ * (queued tiles are numbered in the shape of their scenario).  Results do not
+ * depend on it.  No reference counterpart: the reference's tile is its CUDA
+ * block (propag.rs:482-557); this is the autotuner's tile-shape axis. */
+pg_status pg_ctx_set_tile(pg_ctx *ctx, uint32_t tile_w, uint32_t tile_h);
 
 /* Copies a terrain to the device: fuel codes, slope (rise/run) and aspect
  * (rad), W*H host arrays in the layout of geo.  Returns its id. */

@@ -56,8 +56,7 @@ struct Layout {
 
 Layout layout_for(size_t W, size_t H) {
   const size_t n = W * H;
-  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
-  const size_t nwords = static_cast<size_t>(1) << need_words_log2(ntiles);
+  const size_t nwords = static_cast<size_t>(1) << need_words_log2(max_tiles(W, H));
   Layout L{};
   size_t off = 0;
   L.scen = off; off = align_up(off + scen_buffer_bytes(W, H), 256);
@@ -93,6 +92,7 @@ struct pg_ctx {
   // Scenarios finished since the last pg_ctx_take_finished, oldest first.
   std::vector<int> finished;
   int k_next = 0;
+  int tile_w = pg::kTile;  // tile shape of the scenarios it admits: tile_w x 64 / tile_w
 };
 
 namespace pg {
@@ -333,7 +333,7 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   settings.geo_ref = t.geo;
   settings.max_time = max_time;
   Scen sc;
-  pg_status st = scen_geometry(&settings, &sc);
+  pg_status st = scen_geometry(&settings, &sc, c->tile_w);
   if (st != PG_OK) return st;
   if (max_iterations) sc.max_iters = max_iterations;
   if (sc.max_iters > 0x3fffffffu) return set_error(PG_ERR_INVALID, "max_iterations too large");
@@ -454,6 +454,17 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   return PG_OK;
 }
 
+pg_status pg_ctx_set_tile(pg_ctx *c, uint32_t tile_w, uint32_t tile_h) {
+  if (!c) return set_error(PG_ERR_INVALID, "pg_ctx_set_tile: null context");
+  if (!tile_shape_ok(tile_w, tile_h)) return set_error(PG_ERR_INVALID, "tile shape must be 8x8, 16x4 or 32x2");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (static_cast<int>(tile_w) == c->tile_w) return PG_OK;
+  for (const Scenario &s : c->scen)
+    if (s.used || s.zombie) return set_error(PG_ERR_BUSY, "the tile shape changes only with no scenario admitted");
+  c->tile_w = static_cast<int>(tile_w);
+  return PG_OK;
+}
+
 pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, pg_stats *stats,
                        uint32_t *n_finished) {
   if (!c) return set_error(PG_ERR_INVALID, "pg_batch_run: null context");
@@ -475,6 +486,9 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
     }
   }
   if (!any) return PG_OK;
+  if (tuning && (tuning->tile_w || tuning->tile_h) &&
+      (static_cast<int>(tuning->tile_w) != c->tile_w || static_cast<int>(tuning->tile_h) != 64 / c->tile_w))
+    return set_error(PG_ERR_INVALID, "the tuning's tile shape differs from the context's (pg_ctx_set_tile)");
   if (max_sweeps) k_end = std::min<long long>(k_end, static_cast<long long>(c->k_next) + max_sweeps);
   if (k_end > 0x7fff0000LL) return set_error(PG_ERR_INVALID, "sweep counter exhausted; recreate the context");
 
@@ -497,6 +511,7 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   L.k_begin = c->k_next;
   L.k_end = static_cast<int>(k_end);
   L.cache = 1;
+  L.tw = c->tile_w;
   if (live == 1 && nslots == 1) {
     L.lds_bits_src = c->scen[only].bits;
     L.lds_bits_words = (static_cast<size_t>(c->scen[only].sc.g.W) * c->scen[only].sc.g.H + 31) / 32;

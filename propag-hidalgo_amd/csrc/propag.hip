@@ -168,8 +168,14 @@ __device__ inline void wave_sync() {
 
 // ------------------------------------------------------------------ geometry
 
-constexpr int kHalo = kTile + 2;        // staged edge
-constexpr int kHaloCells = kHalo * kHalo;
+// A tile of TW x (64 / TW) cells, one per lane (lane l: cell (l % TW, l / TW)),
+// staged with a one-cell halo of HW x HH cells.
+template <int TW>
+struct Tile {
+  static constexpr int W = TW, H = kWave / TW;
+  static constexpr int HW = W + 2, HH = H + 2;
+  static constexpr int Cells = HW * HH;
+};
 
 
 // Deterministic double sin/cos: reduction by pi/2 with the two-part constant
@@ -435,7 +441,7 @@ __device__ inline bool line_clear(int x, int y, int rx, int ry, const BitView &b
 }
 
 __device__ inline BitView scen_bits(const Scen &sc, unsigned lds_off) {
-  return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.ntx, sc.nty};
+  return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.bw, sc.bh};
 }
 
 
@@ -463,18 +469,21 @@ struct Args {
   unsigned prof_sweeps;      // rows of prof
 };
 
-constexpr int kPairsMax = kTile * kTile * 8;  // (cell, neighbour) pairs of one tile
+constexpr int kPairsMax = kWave * 8;  // (cell, neighbour) pairs of one tile
 
-struct TileLds {  // one wave's staged tile (8x8 + 1-cell halo)
-  unsigned long long key[kTile * kTile];  // per cell: least candidate key (packed rule)
-  float time[kHaloCells];
-  uint32_t ref[kHaloCells];
-  float rtime[kHaloCells];
-  float ca[kHaloCells], cb[kHaloCells], cr[kHaloCells];  // fire coefficients A, B, R (R = inf: null fire)
+template <int TW>
+struct TileLds {  // one wave's staged tile (TW x 64 / TW cells + 1-cell halo)
+  static constexpr int kCells = (Tile<TW>::Cells + 7) / 8 * 8;
+  unsigned long long key[kWave];  // per cell: least candidate key (packed rule)
+  float time[kCells];
+  uint32_t ref[kCells];
+  float rtime[kCells];
+  float ca[kCells], cb[kCells], cr[kCells];  // fire coefficients A, B, R (R = inf: null fire)
   uint16_t pair[kPairsMax];  // packed rule: cell << 4 | neighbour
-  uint8_t chg[kHaloCells];   // bit 0: changed in sweep k - 1; bit 1: also feeds the rule
+  uint8_t chg[kCells];       // bit 0: changed in sweep k - 1; bit 1: also feeds the rule
 };
-static_assert(sizeof(TileLds) % 8 == 0, "TileLds must keep 8-byte alignment");
+static_assert(sizeof(TileLds<8>) % 8 == 0 && sizeof(TileLds<16>) % 8 == 0 && sizeof(TileLds<32>) % 8 == 0,
+              "TileLds must keep 8-byte alignment");
 
 
 // Per-workgroup profiling line of the grid barrier, 128 bytes each, one
@@ -579,12 +588,12 @@ __device__ inline uint32_t *mark_of(const Scen &sc, int k) { return (k & 1) ? sc
 
 // A list entry is scenario << 32 | xcd << 29 | tile: xcd (0..7) names the
 // XCD whose workgroups run the tile in launches that split work by XCD,
-// the same for all tiles of one 8 x 8-tile block of a scenario, so a tile's
-// halo and its neighbours' stay in one L2 while a fire front spans several
-// XCDs.
+// the same for all tiles of one 64 x 64-cell block of a scenario (the tile
+// with origin (cx, cy) lies in block (cx >> 6, cy >> 6)), so a tile's halo
+// and its neighbours' stay in one L2 while a fire front spans several XCDs.
 constexpr uint32_t kTileMask = 0x1FFFFFFFu;
-__device__ inline uint32_t entry_low(uint32_t si, int X, int Y, uint32_t t) {
-  const uint32_t tag = (static_cast<uint32_t>(X >> 3) + static_cast<uint32_t>(Y >> 3) * 3u + si * 5u) & 7u;
+__device__ inline uint32_t entry_low(uint32_t si, int cx, int cy, uint32_t t) {
+  const uint32_t tag = (static_cast<uint32_t>(cx >> 6) + static_cast<uint32_t>(cy >> 6) * 3u + si * 5u) & 7u;
   return t | (tag << 29);
 }
 // The sub-list an entry goes to.
@@ -622,11 +631,12 @@ struct Mark {
   uint32_t low;  // the list entry's low word (tile and XCD tag)
 };
 
+template <int TW>
 __device__ inline Mark mark_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k1, bool valid, int X, int Y) {
   Mark m{0u, 0u, 0u, valid, 0u};
   if (valid) {
     m.t = static_cast<uint32_t>(X) + static_cast<uint32_t>(Y) * static_cast<uint32_t>(sc.ntx);
-    m.low = entry_low(si, X, Y, m.t);
+    m.low = entry_low(si, X * TW, Y * Tile<TW>::H, m.t);
     m.bit = 1u << (m.t >> sc.nwords_log2);
     m.old = or_fetch(&mark_of(sc, k1)[m.t & ((1u << sc.nwords_log2) - 1u)], m.bit);
   }
@@ -672,7 +682,7 @@ __device__ inline void queue_tile(const TileQueue &q, const Scen &sc, uint32_t s
   const uint32_t bit = 1u << (t >> sc.nwords_log2);
   const uint32_t old = atomicOr(&mark_of(sc, k1)[t & ((1u << sc.nwords_log2) - 1u)], bit);
   if (old & bit) return;
-  const uint32_t low = entry_low(si, X, Y, t);
+  const uint32_t low = entry_low(si, X * sc.tw, Y * sc.th, t);
   const uint32_t x = sub_of(q, low);
   const size_t slot = atomicAdd(count_of(q, k1, x), 1u);
   if (slot < q.cap) list_of(q, k1, x)[slot] = (static_cast<unsigned long long>(si) << 32) | low;
@@ -687,14 +697,15 @@ struct NextTiles {
   int X0, Y0, w, n;  // tile range (n = 0: none)
 };
 
+template <int TW>
 __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
   NextTiles nt{0, 0, 1, 0};
   const uint32_t x0 = __shfl(bx, 0), y0 = __shfl(bx, 1), x1 = __shfl(bx, 2), y1 = __shfl(bx, 3);
   if (x0 <= x1 && y0 <= y1) {  // an empty box is x0 = y0 = 0xFFFFFFFF, x1 = y1 = 0
-    nt.X0 = max(static_cast<int>(x0) - 1, 0) / kTile;
-    nt.Y0 = max(static_cast<int>(y0) - 1, 0) / kTile;
-    const int X1 = min(static_cast<int>(x1) + 1, sc.g.W - 1) / kTile;
-    const int Y1 = min(static_cast<int>(y1) + 1, sc.g.H - 1) / kTile;
+    nt.X0 = max(static_cast<int>(x0) - 1, 0) / TW;
+    nt.Y0 = max(static_cast<int>(y0) - 1, 0) / Tile<TW>::H;
+    const int X1 = min(static_cast<int>(x1) + 1, sc.g.W - 1) / TW;
+    const int Y1 = min(static_cast<int>(y1) + 1, sc.g.H - 1) / Tile<TW>::H;
     nt.w = X1 - nt.X0 + 1;
     nt.n = nt.w * (Y1 - nt.Y0 + 1);
   }
@@ -709,42 +720,48 @@ __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
 // in the tile or its one-cell ring.  So with F = the changed cells and the
 // unchanged cells of the tile and ring whose reference is a changed
 // neighbour, a neighbour tile is queued exactly when some cell of F lies
-// next to it (tile-local x <= 0 for the tile on the left, >= kTile - 1 for
-// the one on the right, the same for y).  Returns the 9-bit set (bit
+// next to it (tile-local x <= 0 for the tile on the left, >= TW - 1 for the
+// one on the right, the same for y).  Returns the 9-bit set (bit
 // (dy + 1) * 3 + dx + 1), uniform over the wave; called by all lanes with
-// `imp` the ballot of the tile's changed cells (lane l = cell (l % 8, l / 8)).
-template <bool kCache>
-__device__ inline uint32_t next_neighbours(const TileLds &s, int ox, int oy, int lx, int ly, bool improved,
+// `imp` the ballot of the tile's changed cells (lane l = cell (l % TW, l / TW)).
+template <bool kCache, int TW>
+__device__ inline uint32_t next_neighbours(const TileLds<TW> &s, int ox, int oy, int lx, int ly, bool improved,
                                            unsigned long long imp, int lane) {
+  constexpr int TH = Tile<TW>::H, HW = Tile<TW>::HW;
   // A cell at tile-local (cx, cy) whose reference is a changed neighbour in
   // the tile.
   auto near_dep = [&](int cx, int cy, uint32_t ref) {
     const int rx = ref_x<kCache>(ref) - ox, ry = ref_y<kCache>(ref) - oy;
     const bool next = abs(rx - cx) <= 1 && abs(ry - cy) <= 1 && (rx != cx || ry != cy);
-    return next && rx >= 0 && ry >= 0 && rx < kTile && ry < kTile && ((imp >> (rx + ry * kTile)) & 1ull);
+    return next && rx >= 0 && ry >= 0 && rx < TW && ry < TH && ((imp >> (rx + ry * TW)) & 1ull);
   };
   // Bits of the neighbour tiles next to tile-local (cx, cy).
   auto sides = [](int cx, int cy) {
-    const uint32_t xs = 2u | (cx <= 0 ? 1u : 0u) | (cx >= kTile - 1 ? 4u : 0u);
-    const uint32_t ys = 2u | (cy <= 0 ? 1u : 0u) | (cy >= kTile - 1 ? 4u : 0u);
+    const uint32_t xs = 2u | (cx <= 0 ? 1u : 0u) | (cx >= TW - 1 ? 4u : 0u);
+    const uint32_t ys = 2u | (cy <= 0 ? 1u : 0u) | (cy >= TH - 1 ? 4u : 0u);
     return xs * (ys & 1u) | ((xs * ((ys >> 1) & 1u)) << 3) | ((xs * ((ys >> 2) & 1u)) << 6);
   };
   uint32_t m = 0;
   const int cx = lx - 1, cy = ly - 1;
-  if (improved || near_dep(cx, cy, s.ref[lx + ly * kHalo])) m = sides(cx, cy);
-  // ring cell of this lane: top and bottom rows (lanes 0..2*kHalo-1), then
-  // the left and right columns
-  if (lane < 4 * kTile + 4) {
-    int rx, ry;
-    if (lane < 2 * kHalo) {
-      rx = lane % kHalo - 1;
-      ry = lane < kHalo ? -1 : kTile;
-    } else {
-      const int l = lane - 2 * kHalo;
-      rx = l < kTile ? -1 : kTile;
-      ry = l % kTile;
+  if (improved || near_dep(cx, cy, s.ref[lx + ly * HW])) m = sides(cx, cy);
+  // the one-cell ring, a cell per lane per round: the top and bottom rows
+  // (HW cells each), then the left and right columns (TH cells each)
+  constexpr int kRing = 2 * HW + 2 * TH;
+#pragma unroll
+  for (int r0 = 0; r0 < kRing; r0 += kWave) {
+    const int l = r0 + lane;
+    if (l < kRing) {
+      int rx, ry;
+      if (l < 2 * HW) {
+        rx = l % HW - 1;
+        ry = l < HW ? -1 : TH;
+      } else {
+        const int c = l - 2 * HW;
+        rx = c < TH ? -1 : TW;
+        ry = c % TH;
+      }
+      if (near_dep(rx, ry, s.ref[(rx + 1) + (ry + 1) * HW])) m |= sides(rx, ry);
     }
-    if (near_dep(rx, ry, s.ref[(rx + 1) + (ry + 1) * kHalo])) m |= sides(rx, ry);
   }
   uint32_t q = 0;
 #pragma unroll
@@ -752,19 +769,33 @@ __device__ inline uint32_t next_neighbours(const TileLds &s, int ox, int oy, int
   return q;
 }
 
+// Ballot masks of a tile's two first / last columns and rows.
+template <int TW>
+constexpr unsigned long long two_cols() {
+  unsigned long long m = 0;
+  for (int y = 0; y < Tile<TW>::H; ++y) m |= 3ull << (y * TW);
+  return m;
+}
+template <int TW>
+constexpr unsigned long long two_rows() {
+  return 2 * TW >= 64 ? ~0ull : (1ull << (2 * TW)) - 1ull;
+}
+
 // The same set widened to what the tile's ballot alone shows: a neighbour
 // tile is queued if a changed cell lies within two cells of it (a superset
 // of next_neighbours, without its reads and ballots: the one-scenario
 // kernel with the bitset in LDS runs its sweeps as one dependent chain, where
 // those cost more than the extra visits they save).
+template <int TW>
 __device__ inline uint32_t near_band(unsigned long long imp) {
-  constexpr unsigned long long kCol01 = 0x0303030303030303ull, kRow01 = 0xFFFFull;
+  constexpr unsigned long long kLeft = two_cols<TW>(), kRight = two_cols<TW>() << (TW - 2);
+  constexpr unsigned long long kTop = two_rows<TW>(), kBottom = two_rows<TW>() << (TW * (Tile<TW>::H - 2));
   uint32_t q = 0;
 #pragma unroll
   for (int b = 0; b < 9; ++b) {
     const int dx = b % 3 - 1, dy = b / 3 - 1;
-    const unsigned long long selx = dx < 0 ? kCol01 : (dx > 0 ? kCol01 << (kTile - 2) : ~0ull);
-    const unsigned long long sely = dy < 0 ? kRow01 : (dy > 0 ? kRow01 << (kTile * (kTile - 2)) : ~0ull);
+    const unsigned long long selx = dx < 0 ? kLeft : (dx > 0 ? kRight : ~0ull);
+    const unsigned long long sely = dy < 0 ? kTop : (dy > 0 ? kBottom : ~0ull);
     q |= (imp & selx & sely) != 0ull ? 1u << b : 0u;
   }
   return q;
@@ -773,24 +804,26 @@ __device__ inline uint32_t near_band(unsigned long long imp) {
 // Issues the marks of the queued neighbour tiles `nb` (lanes 0..8) and of
 // the box of far dependents (box tile q in lane 9 + q of round 0 and lane
 // (q - 55) % 64 of round 1 + (q - 55) / 64).
+template <int TW>
 __device__ inline Mark next_round0_issue(const TileQueue &q, const Scen &sc, uint32_t si, int k, int tx, int ty,
                                          const NextTiles &nt, uint32_t nb, int lane) {
   if (lane < 9) {
     const int dx = lane % 3 - 1, dy = lane / 3 - 1;
     const int X = tx + dx, Y = ty + dy;
-    return mark_issue(q, sc, si, k + 1, ((nb >> lane) & 1u) && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X, Y);
+    return mark_issue<TW>(q, sc, si, k + 1, ((nb >> lane) & 1u) && X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty, X,
+                          Y);
   }
   const int b = lane - 9;
-  return mark_issue(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+  return mark_issue<TW>(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
 }
 
 // Rounds 1.. of the box; returns the number of tiles marked in all rounds.
-template <bool kHold>
+template <bool kHold, int TW>
 __device__ inline uint32_t next_rest(const TileQueue &q, const Scen &sc, uint32_t si, int k, const NextTiles &nt,
                                      WalkQueue &wq, int lane) {
   for (int b0 = kWave - 9; b0 < nt.n; b0 += kWave) {
     const int b = b0 + lane;
-    const Mark mk = mark_issue(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
+    const Mark mk = mark_issue<TW>(q, sc, si, k + 1, b < nt.n, nt.X0 + b % nt.w, nt.Y0 + b / nt.w);
     if (kHold)
       mark_hold(q, wq, si, k + 1, mk, lane);
     else
@@ -825,9 +858,11 @@ __device__ inline float from_order_bits(uint32_t o) {
 // are evaluated: their (cell, neighbour) pairs are packed over the wave's
 // lanes and each cell's least key (order-mapped time, neighbour index,
 // reference-or-neighbour bit) is an LDS atomic minimum.
-__device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, int lx, int ly, int gx, int gy,
+template <int TW>
+__device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy, int lx, int ly, int gx, int gy,
                                    int lane, bool need, Counters &cnt, bool &improved, float &best_t_out,
                                    uint32_t &best_ref_out) {
+  constexpr int HW = Tile<TW>::HW;
   uint32_t m = 0;
   if (need) {
     ++cnt.relax;
@@ -837,7 +872,7 @@ __device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, in
       const int i = d % 3 - 1, j = d / 3 - 1;
       const int nx = gx + i, ny = gy + j;
       const bool inb = nx >= 0 && ny >= 0 && nx < g.W && ny < g.H;
-      m |= (inb && s.chg[(lx + i) + (ly + j) * kHalo] == 3) ? (1u << d) : 0u;
+      m |= (inb && s.chg[(lx + i) + (ly + j) * HW] == 3) ? (1u << d) : 0u;
     }
   }
   // Exclusive wave scan of the pair counts (0..8), one ballot per bit.
@@ -859,10 +894,10 @@ __device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, in
     if (e < total) {
       const uint32_t pr = s.pair[e];
       const int cl = static_cast<int>(pr >> 4), d = static_cast<int>(pr & 15u);
-      const int cx = (cl & (kTile - 1)) + 1, cy = cl / kTile + 1;
+      const int cx = cl % TW + 1, cy = cl / TW + 1;
       const int tx = ox + cx - 1, ty = oy + cy - 1;
       const int i = d % 3 - 1, j = d / 3 - 1;
-      const int nl = (cx + i) + (cy + j) * kHalo;
+      const int nl = (cx + i) + (cy + j) * HW;
       const uint32_t ref = s.ref[nl];
       const int rfx = ref_x<true>(ref), rfy = ref_y<true>(ref);
       if (rfx != tx || rfy != ty) {  // a neighbour referencing this cell offers nothing
@@ -879,7 +914,7 @@ __device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, in
   }
   wave_sync();
   if (need) {
-    const int me = lx + ly * kHalo;
+    const int me = lx + ly * HW;
     const float own_t = s.time[me];
     const unsigned long long key = s.key[lane];
     const float cand = from_order_bits(static_cast<uint32_t>(key >> 5));
@@ -887,20 +922,21 @@ __device__ inline void packed_rule(TileLds &s, const Geom &g, int ox, int oy, in
     if (improved) {
       const int d = static_cast<int>((key >> 1) & 15u);
       const int i = d % 3 - 1, j = d / 3 - 1;
-      const uint32_t ref = s.ref[(lx + i) + (ly + j) * kHalo];
+      const uint32_t ref = s.ref[(lx + i) + (ly + j) * HW];
       best_t_out = cand;
       best_ref_out = (key & 1u) ? make_ref(ref_x<true>(ref), ref_y<true>(ref)) : make_ref(gx + i, gy + j);
     }
   }
 }
 
-// One sweep over one tile by one wave: lane l owns cell (l % 8, l / 8) of
-// the 8x8 tile.  Returns, uniformly over the wave, whether any cell
-// committed an improvement.
-template <bool kCache, bool kLdsBits>
-__device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned lds_off, int tile, int k, TileLds &s,
-                          WalkQueue &wq, Counters &cnt) {
+// One sweep over one tile by one wave: lane l owns cell (l % TW, l / TW) of
+// the TW x (64 / TW) tile.  Returns, uniformly over the wave, whether any
+// cell committed an improvement.
+template <bool kCache, bool kLdsBits, int TW>
+__device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned lds_off, int tile, int k,
+                          TileLds<TW> &s, WalkQueue &wq, Counters &cnt) {
   constexpr bool kDefer = defer_walks<kCache, kLdsBits>();
+  constexpr int TH = Tile<TW>::H, HW = Tile<TW>::HW, kHaloCells = Tile<TW>::Cells;
   // Descriptor fields in registers once (sc lives in global memory, so each
   // use of a field after a store would otherwise reload it).
   const Geom g = sc.g;
@@ -913,7 +949,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   const uint2 *__restrict__ st_in = (k & 1) ? sc.st[1] : sc.st[0];
   uint2 *__restrict__ st_out = (k & 1) ? sc.st[0] : sc.st[1];
   const int tx = tile % sc.ntx, ty = tile / sc.ntx;
-  const int ox = tx * kTile, oy = ty * kTile;
+  const int ox = tx * TW, oy = ty * TH;
   const int lane = threadIdx.x & (kWave - 1);
   uint64_t tp0 = a.prof ? wall_clock64() : 0;
 
@@ -933,7 +969,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
 #pragma unroll
   for (int it = 0; it < kPer; ++it) {
     const int h = lane + it * kWave;
-    const int gx = ox + h % kHalo - 1, gy = oy + h / kHalo - 1;
+    const int gx = ox + h % HW - 1, gy = oy + h / HW - 1;
     const bool inb = h < kHaloCells && gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
     const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
     if (a.cached_stage) {
@@ -983,14 +1019,14 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   uint64_t tp1 = a.prof ? wall_clock64() : 0;
   uint64_t walk_ticks = 0;
 
-  const int lx = (lane & (kTile - 1)) + 1, ly = lane / kTile + 1;
+  const int lx = lane % TW + 1, ly = lane / TW + 1;
   const int gx = ox + lx - 1, gy = oy + ly - 1;
   bool need = false;
   if (gx < g.W && gy < g.H) {
 #pragma unroll
     for (int j = -1; j < 2; ++j)
 #pragma unroll
-      for (int i = -1; i < 2; ++i) need = need || (s.chg[(lx + i) + (ly + j) * kHalo] & 1);
+      for (int i = -1; i < 2; ++i) need = need || (s.chg[(lx + i) + (ly + j) * HW] & 1);
   }
   bool improved = false;
   bool deferred = false;  // improved with a new reference whose walk is queued
@@ -1000,10 +1036,10 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   float best_t_out = 0.0f;
   uint32_t best_ref_out = 0u;
   if constexpr (kCache) {
-    packed_rule(s, g, ox, oy, lx, ly, gx, gy, lane, need, cnt, improved, best_t_out, best_ref_out);
+    packed_rule<TW>(s, g, ox, oy, lx, ly, gx, gy, lane, need, cnt, improved, best_t_out, best_ref_out);
   } else if (need) {
     ++cnt.relax;
-    const int me = lx + ly * kHalo;
+    const int me = lx + ly * HW;
     const float own_t = s.time[me];
     float best_t = own_t;
     uint32_t best_ref = s.ref[me];
@@ -1016,7 +1052,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         if (i == 0 && j == 0) continue;
         const int nx = gx + i, ny = gy + j;
         if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H) continue;
-        const int nl = (lx + i) + (ly + j) * kHalo;
+        const int nl = (lx + i) + (ly + j) * HW;
         const float n_t = s.time[nl];
         if (!(n_t < g.max_time)) continue;
         const float kr = s.cr[nl];
@@ -1055,7 +1091,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   const bool changed = imp != 0ull;
   const bool idle = __ballot(need) == 0ull;  // (a ballot of every lane)
   if (lane == 0 && idle) ++cnt.idle;
-  if (a.prof && idle) {
+  if (TW == kTile && a.prof && idle) {
     // Diagnostics: nothing in an idle tile's halo changed; if nothing two
     // cells from the tile changed either, only a far-dependents box queued
     // it.  Lanes 0..43 read the ring of cells two cells from the tile.
@@ -1084,12 +1120,13 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     // some cell's far reference.
     const unsigned long long refs =
         static_cast<unsigned long long>(__shfl(bx, 4)) | (static_cast<unsigned long long>(__shfl(bx, 5)) << 32);
-    if (imp & refs) nt = next_tiles(sc, bx);
-    const uint32_t nb = kLdsBits ? near_band(imp) : next_neighbours<kCache>(s, ox, oy, lx, ly, improved, imp, lane);
-    mk0 = next_round0_issue(a.q, sc, si, k, tx, ty, nt, nb, lane);
+    if (imp & refs) nt = next_tiles<TW>(sc, bx);
+    const uint32_t nb =
+        kLdsBits ? near_band<TW>(imp) : next_neighbours<kCache, TW>(s, ox, oy, lx, ly, improved, imp, lane);
+    mk0 = next_round0_issue<TW>(a.q, sc, si, k, tx, ty, nt, nb, lane);
   }
   if (need) {
-    const int me = lx + ly * kHalo;
+    const int me = lx + ly * HW;
     const size_t gi = static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W;
     float best_t = best_t_out;
     uint32_t best_ref = best_ref_out;
@@ -1110,8 +1147,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       if (a.prof) walk_ticks += wall_clock64() - tw;
       const int brx = ref_x<kCache>(best_ref), bry = ref_y<kCache>(best_ref);
       far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
-      rtile = static_cast<uint32_t>(brx / kTile) + static_cast<uint32_t>(bry / kTile) * static_cast<uint32_t>(sc.ntx);
-      rbit = static_cast<uint32_t>((brx & 7) | ((bry & 7) << 3));
+      rtile = static_cast<uint32_t>(brx / TW) + static_cast<uint32_t>(bry / TH) * static_cast<uint32_t>(sc.ntx);
+      rbit = static_cast<uint32_t>(brx % TW + (bry % TH) * TW);
       if (!deferred) st_state(&st_out[gi], pack_state(best_t, best_ref));
       st_i32(&stamp[gi], k);
       ++cnt.improve;
@@ -1142,7 +1179,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       mark_hold(a.q, wq, si, k + 1, mk0, lane);
     else
       mark_commit(a.q, si, k + 1, mk0, lane);
-    const uint32_t marks = next_rest<kDefer>(a.q, sc, si, k, nt, wq, lane);
+    const uint32_t marks = next_rest<kDefer, TW>(a.q, sc, si, k, nt, wq, lane);
     if (lane == 0) cnt.marks += marks;
     // Register this tile as a dependent of each new far reference's tile
     // and mark the reference cells, one set of atomics per distinct
@@ -1163,16 +1200,16 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[rt0]);
         if (lo) or_relaxed(r, lo);
         if (hi) or_relaxed(r + 1, hi);
-        // the group's cells' bounding box, from the ballot (lane = x + 8 y)
-        unsigned long long cols = gm | (gm >> 32);
-        cols |= cols >> 16;
-        cols |= cols >> 8;
-        const uint32_t c8 = static_cast<uint32_t>(cols & 0xFFull);
+        // the group's cells' bounding box, from the ballot (lane = x + TW y)
+        unsigned long long cols = gm;
+#pragma unroll
+        for (int sh = 32; sh >= TW; sh >>= 1) cols |= cols >> sh;
+        cols &= TW == 64 ? ~0ull : (1ull << TW) - 1ull;
         uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt0]);
-        min_relaxed(b + 0, static_cast<uint32_t>(ox + __builtin_ctz(c8)));
-        min_relaxed(b + 1, static_cast<uint32_t>(oy + (__builtin_ctzll(gm) >> 3)));
-        max_relaxed(b + 2, static_cast<uint32_t>(ox + 31 - __builtin_clz(c8)));
-        max_relaxed(b + 3, static_cast<uint32_t>(oy + ((63 - __builtin_clzll(gm)) >> 3)));
+        min_relaxed(b + 0, static_cast<uint32_t>(ox + __builtin_ctzll(cols)));
+        min_relaxed(b + 1, static_cast<uint32_t>(oy + __builtin_ctzll(gm) / TW));
+        max_relaxed(b + 2, static_cast<uint32_t>(ox + 63 - __builtin_clzll(cols)));
+        max_relaxed(b + 3, static_cast<uint32_t>(oy + (63 - __builtin_clzll(gm)) / TW));
       }
     }
   }
@@ -1198,8 +1235,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
 // tile's mark, skips released scenarios and those past their sweep budget
 // (flagging the latter as capped), runs the tile.  Returns whether it
 // changed.
-template <bool kCache, bool kLdsBits>
-__device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned lds_off, int k, TileLds &s,
+template <bool kCache, bool kLdsBits, int TW>
+__device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned lds_off, int k, TileLds<TW> &s,
                                  WalkQueue &wq, Counters &cnt) {
   const unsigned lane = threadIdx.x & (kWave - 1);
   const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
@@ -1216,7 +1253,7 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
     return false;
   }
   if (tile >= static_cast<uint32_t>(sc.ntx) * static_cast<uint32_t>(sc.nty)) return false;
-  const bool tc = wave_tile<kCache, kLdsBits>(a, sc, si, lds_off, static_cast<int>(tile), k, s, wq, cnt);
+  const bool tc = wave_tile<kCache, kLdsBits, TW>(a, sc, si, lds_off, static_cast<int>(tile), k, s, wq, cnt);
   if (tc && lane == 0) st_i32(&a.out[si].last_change, k);
   return tc;
 }
@@ -1227,8 +1264,8 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
 // leaves waves idle at the barrier); the next entry's load is issued before
 // the current tile runs.  Entries fill slots from 0, so a wave stops at the
 // first empty slot it draws.
-template <bool kCache, bool kLdsBits>
-__device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds &s, DeferLds *dq,
+template <bool kCache, bool kLdsBits, int TW>
+__device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds<TW> &s, DeferLds *dq,
                                Counters &cnt) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
@@ -1256,7 +1293,7 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
       if (lane == 0) st_u64(&list[e], kNoEntry);  // written again only four sweeps later
       size_t e2;
       const unsigned long long v2 = draw(e2);  // in flight while this tile runs
-      changed |= run_entry<kCache, kLdsBits>(a, v, lds_off, k, s, wq, cnt);
+      changed |= run_entry<kCache, kLdsBits, TW>(a, v, lds_off, k, s, wq, cnt);
       v = v2;
       e = e2;
     }
@@ -1276,8 +1313,8 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
 // ends the wave's share.  Tiles of a scenario past its sweep budget are
 // skipped and the scenario flagged as capped.  Returns whether any tile of
 // this wave changed.
-template <bool kCache, bool kLdsBits>
-__device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds &s,
+template <bool kCache, bool kLdsBits, int TW>
+__device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds<TW> &s,
                            DeferLds *dq, Counters &cnt) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   const unsigned lane = threadIdx.x & (kWave - 1);
@@ -1293,7 +1330,7 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
       for (unsigned long long r = m; r;) {
         const int l = __builtin_ctzll(r);
         r &= r - 1;
-        changed |= run_entry<kCache, kLdsBits>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
+        changed |= run_entry<kCache, kLdsBits, TW>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
       }
       if (m != ~0ull) break;
     }
@@ -1471,7 +1508,7 @@ constexpr int persistent_threads() { return kLdsBits ? 512 : 1024; }
 // List entry e belongs to workgroup e mod gridDim.x, so tiles queued
 // together (neighbours on the fire front) land on different CUs, and each
 // workgroup's waves draw its entries dynamically (sweep_wave_dyn).
-template <bool kCache, bool kLdsBits>
+template <bool kCache, bool kLdsBits, int TW>
 __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_persistent(Args a) {
   // Per-sweep workgroup flags, double-buffered by sweep parity: sweep k
   // writes [k & 1], thread 0 clears it after barrier k, and it is written
@@ -1485,8 +1522,8 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   // word address needs no base; then nw tile slices and (deferred walks) nw
   // blocks of deferred work (walk queue and held list entries).
   const unsigned bits_words = kLdsBits ? (a.lds_words + 3u) & ~3u : 0u;  // 16-byte aligned slices
-  TileLds *slices = reinterpret_cast<TileLds *>(dyn_lds + bits_words);
-  TileLds &s = slices[wave];
+  TileLds<TW> *slices = reinterpret_cast<TileLds<TW> *>(dyn_lds + bits_words);
+  TileLds<TW> &s = slices[wave];
   DeferLds *dq = reinterpret_cast<DeferLds *>(slices + nw) + wave;
   const unsigned lds_off = 0;
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
@@ -1499,7 +1536,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   Counters cnt{};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
-    const bool changed = sweep_wave_dyn<kCache, kLdsBits>(a, s_next[k & 1], lds_off, k, s, dq, cnt);
+    const bool changed = sweep_wave_dyn<kCache, kLdsBits, TW>(a, s_next[k & 1], lds_off, k, s, dq, cnt);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
@@ -1524,9 +1561,9 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
 }
 
 // Plain form: one launch per sweep (the reference's relaunch structure).
-template <bool kCache>
+template <bool kCache, int TW>
 __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
-  __shared__ TileLds slices[256 / kWave];
+  __shared__ TileLds<TW> slices[256 / kWave];
   __shared__ DeferLds dq[256 / kWave];
   __shared__ unsigned s_changed;
   const int wave = threadIdx.x / kWave;
@@ -1535,7 +1572,7 @@ __global__ __launch_bounds__(256) void propagate_sweep(Args a, int k) {
   Counters cnt{};
   const unsigned nw = blockDim.x / kWave;
   const unsigned gwave = wave * gridDim.x + blockIdx.x, nwaves = gridDim.x * nw;
-  const bool changed = sweep_wave<kCache, false>(a, gwave, nwaves, 0, k, slices[wave], &dq[wave], cnt);
+  const bool changed = sweep_wave<kCache, false, TW>(a, gwave, nwaves, 0, k, slices[wave], &dq[wave], cnt);
   if ((threadIdx.x & (kWave - 1)) == 0 && changed) s_changed = 1u;
   __syncthreads();
   if (threadIdx.x == 0 && s_changed) add_relaxed(&a.ctl->changes.v[k & 1], 1u);
@@ -1616,16 +1653,16 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
     sc.st[1][i] = pack_state(t, ref);
     sc.stamp[i] = burned ? sc.k0 - 1 : kStampNever;
     if (burned) {
-      const int tx = x / kTile, ty = y / kTile;
+      const int tx = x / sc.tw, ty = y / sc.th;
       for (int q = 0; q < 9; ++q) {
         const int X = tx + q % 3 - 1, Y = ty + q / 3 - 1;
         if (X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty) queue_tile(tq, sc, si, sc.k0, X, Y);
       }
       if (abs(static_cast<int>(rx[i]) - x) > 1 || abs(static_cast<int>(ry[i]) - y) > 1) {
-        const uint32_t rb = (rx[i] & 7u) | ((ry[i] & 7u) << 3);
-        or_relaxed(reinterpret_cast<uint32_t *>(&sc.isref[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]) + (rb >> 5),
-                   1u << (rb & 31u));
-        uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[(rx[i] / kTile) + (ry[i] / kTile) * sc.ntx]);
+        const uint32_t rb = rx[i] % sc.tw + (ry[i] % sc.th) * sc.tw;
+        const size_t rt = static_cast<size_t>(rx[i] / sc.tw) + static_cast<size_t>(ry[i] / sc.th) * sc.ntx;
+        or_relaxed(reinterpret_cast<uint32_t *>(&sc.isref[rt]) + (rb >> 5), 1u << (rb & 31u));
+        uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt]);
         min_relaxed(b + 0, static_cast<uint32_t>(x));
         min_relaxed(b + 1, static_cast<uint32_t>(y));
         max_relaxed(b + 2, static_cast<uint32_t>(x));
@@ -1903,16 +1940,20 @@ pg_status launch_graph_for(PerDevice *pd, void (*kern)(Args), unsigned groups, u
 
 // ------------------------------------------------------------- scenarios
 
+// Buffers of one scenario, for any tile shape: boxes and far-reference
+// bits for the most tiles a shape cuts the raster into, the blocked bitset
+// in 8x8 blocks.
 size_t scen_buffer_bytes(size_t W, size_t H) {
   const size_t n = W * H;
-  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+  const size_t ntiles = max_tiles(W, H);
+  const size_t nblocks = ((W + 7) / 8) * ((H + 7) / 8);
   size_t off = 0;
   off = align_up(off + n * sizeof(Coef), 256);
   off = align_up(off + n * sizeof(uint2), 256);
   off = align_up(off + n * sizeof(uint2), 256);
   off = align_up(off + n * sizeof(int), 256);
   off = align_up(off + ntiles * sizeof(uint4), 256);
-  off = align_up(off + ntiles * sizeof(unsigned long long), 256);
+  off = align_up(off + nblocks * sizeof(unsigned long long), 256);
   off = align_up(off + ntiles * sizeof(unsigned long long), 256);
   return off;
 }
@@ -1926,15 +1967,18 @@ ScenBuffers scen_carve(void *base, size_t W, size_t H) {
   b.st0 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.st1 = reinterpret_cast<uint2 *>(p + off); off = align_up(off + n * sizeof(uint2), 256);
   b.stamp = reinterpret_cast<int *>(p + off); off = align_up(off + n * sizeof(int), 256);
-  const size_t ntiles = ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+  const size_t ntiles = max_tiles(W, H);
+  const size_t nblocks = ((W + 7) / 8) * ((H + 7) / 8);
   b.box = reinterpret_cast<uint4 *>(p + off); off = align_up(off + ntiles * sizeof(uint4), 256);
-  b.blk = reinterpret_cast<unsigned long long *>(p + off); off = align_up(off + ntiles * 8, 256);
+  b.blk = reinterpret_cast<unsigned long long *>(p + off); off = align_up(off + nblocks * 8, 256);
   b.isref = reinterpret_cast<unsigned long long *>(p + off);
   return b;
 }
 
-pg_status scen_geometry(const Settings *settings, Scen *sc) {
+pg_status scen_geometry(const Settings *settings, Scen *sc, int tw) {
   if (!settings) return set_error(PG_ERR_INVALID, "settings is NULL");
+  if (!tile_shape_ok(static_cast<uint32_t>(tw), static_cast<uint32_t>(kWave / tw)))
+    return set_error(PG_ERR_INVALID, "tile shape must be 8x8, 16x4 or 32x2");
   const uint32_t W = settings->geo_ref.width, H = settings->geo_ref.height;
   if (W == 0 || H == 0) return set_error(PG_ERR_INVALID, "raster has no cells");
   if (W > 0xFFFFu || H > 0xFFFFu)
@@ -1953,8 +1997,12 @@ pg_status scen_geometry(const Settings *settings, Scen *sc) {
   sc->g.north_up = gdy < 0.0f;
   sc->g.square = fabsf(gdx) == fabsf(gdy);
   sc->cache = (W <= 0x7fffu && H <= 0x7fffu) ? 1 : 0;
-  sc->ntx = static_cast<int>((W + kTile - 1) / kTile);
-  sc->nty = static_cast<int>((H + kTile - 1) / kTile);
+  sc->tw = tw;
+  sc->th = kWave / tw;
+  sc->ntx = static_cast<int>((W + sc->tw - 1) / sc->tw);
+  sc->nty = static_cast<int>((H + sc->th - 1) / sc->th);
+  sc->bw = static_cast<int>((W + 7) / 8);
+  sc->bh = static_cast<int>((H + 7) / 8);
   sc->nwords_log2 = need_words_log2(static_cast<size_t>(sc->ntx) * sc->nty);
   sc->max_iters = 64u * (W + H) + 1024u;
   return PG_OK;
@@ -1978,9 +2026,9 @@ pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenO
   hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, err, static_cast<size_t>(1), 0u);
   hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, sc.g, speed, az, ecc,
                      boundaries);
-  hipLaunchKernelGGL(block_bits, grid_for(ntiles), dim3(256), 0, stream, boundaries,
-                     const_cast<unsigned long long *>(sc.blk), sc.g.W, sc.g.H, static_cast<uint32_t>(sc.ntx),
-                     static_cast<uint32_t>(sc.nty));
+  hipLaunchKernelGGL(block_bits, grid_for(static_cast<size_t>(sc.bw) * sc.bh), dim3(256), 0, stream, boundaries,
+                     const_cast<unsigned long long *>(sc.blk), sc.g.W, sc.g.H, static_cast<uint32_t>(sc.bw),
+                     static_cast<uint32_t>(sc.bh));
   hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, sc, si, q, err, speed, az, ecc, time, refs_x,
                      refs_y);
   PG_HIP(hipGetLastError());
@@ -2041,12 +2089,34 @@ size_t launch_ws_bytes(unsigned cu, bool profile) {
   return off;
 }
 
+// The kernels and LDS slice of a tile width.
+static size_t tile_lds_bytes(int tw) {
+  return tw == 16 ? sizeof(TileLds<16>) : tw == 32 ? sizeof(TileLds<32>) : sizeof(TileLds<8>);
+}
+template <int TW>
+static void (*persistent_for(bool cache, bool lds))(Args) {
+  return cache ? (lds ? propagate_persistent<true, true, TW> : propagate_persistent<true, false, TW>)
+               : (lds ? propagate_persistent<false, true, TW> : propagate_persistent<false, false, TW>);
+}
+static void (*persistent_kernel(int tw, bool cache, bool lds))(Args) {
+  return tw == 16 ? persistent_for<16>(cache, lds) : tw == 32 ? persistent_for<32>(cache, lds)
+                                                             : persistent_for<8>(cache, lds);
+}
+template <int TW>
+static void (*sweep_for(bool cache))(Args, int) {
+  return cache ? propagate_sweep<true, TW> : propagate_sweep<false, TW>;
+}
+static void (*sweep_kernel(int tw, bool cache))(Args, int) {
+  return tw == 16 ? sweep_for<16>(cache) : tw == 32 ? sweep_for<32>(cache) : sweep_for<8>(cache);
+}
+
 pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tuning, pg_stats *stats,
                      LaunchResult *res, hipStream_t stream) {
   pg_tuning tn{};
   if (tuning) tn = *tuning;
-  if (tn.tile_w && tn.tile_w != kTile) return set_error(PG_ERR_INVALID, "tile_w must be 8");
-  if (tn.tile_h && tn.tile_h != kTile) return set_error(PG_ERR_INVALID, "tile_h must be 8");
+  const int tw = L.tw ? L.tw : kTile;
+  if ((tn.tile_w || tn.tile_h) && (static_cast<int>(tn.tile_w) != tw || static_cast<int>(tn.tile_h) != kWave / tw))
+    return set_error(PG_ERR_INVALID, "the tuning's tile shape differs from the scenarios' tiling");
   if (L.k_end <= L.k_begin) return set_error(PG_ERR_INVALID, "empty sweep range");
   // A caller's workspace has no room for profile rows and no cached graphs.
   if (L.ws && (tn.profile || tn.mode == 2))
@@ -2128,11 +2198,9 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     launched_groups = grid.x;
     a.lds_bits = 0;
     a.cached_stage = 0;
+    void (*sweep)(Args, int) = sweep_kernel(tw, L.cache != 0);
     for (int k = L.k_begin;; ++k) {
-      if (L.cache)
-        hipLaunchKernelGGL(propagate_sweep<true>, grid, dim3(256), 0, stream, a, k);
-      else
-        hipLaunchKernelGGL(propagate_sweep<false>, grid, dim3(256), 0, stream, a, k);
+      hipLaunchKernelGGL(sweep, grid, dim3(256), 0, stream, a, k);
       hipLaunchKernelGGL(sweep_book, dim3(1), dim3(1), 0, stream, a, k);
       PG_HIP(hipGetLastError());
       PG_HIP(hipMemcpyAsync(book, &a.ctl->book, sizeof(book), hipMemcpyDeviceToHost, stream));
@@ -2164,18 +2232,19 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     const size_t static_lds = 256;
     int waves = 16;
     a.lds_bits = 0;
-    if (L.nscen == 1 && L.lds_bits_src && bits_bytes + static_lds + sizeof(TileLds) * 4 <= dev->lds_per_block) {
+    const size_t slice = tile_lds_bytes(tw);
+    if (L.nscen == 1 && L.lds_bits_src && bits_bytes + static_lds + slice * 4 <= dev->lds_per_block) {
       a.lds_bits = 1;
       a.lds_src = L.lds_bits_src;
       a.lds_words = static_cast<unsigned>(L.lds_bits_words);
-      waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / sizeof(TileLds));
+      waves = static_cast<int>((dev->lds_per_block - bits_bytes - static_lds) / slice);
     }
     // per-wave LDS: the tile slice, plus the deferred-work block when walks
     // are deferred (line-of-sight flags cached and the bitset read from L2)
     // With the tiles split by XCD, a scenario's halos are staged through one
     // L2: stage them with cacheable loads (the barrier then acquires).
     a.cached_stage = (L.q.nsub > 1 && !a.lds_bits) ? 1 : 0;
-    const size_t wave_lds = sizeof(TileLds) + ((L.cache && !a.lds_bits) ? sizeof(DeferLds) : 0);
+    const size_t wave_lds = slice + ((L.cache && !a.lds_bits) ? sizeof(DeferLds) : 0);
     const int max_waves = (a.lds_bits ? persistent_threads<true>() : persistent_threads<false>()) / kWave;
     if (tn.waves_per_group) {
       // with the bitset in LDS, at most the slices that fit beside it
@@ -2184,8 +2253,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     if (waves > max_waves) waves = max_waves;
     const size_t shmem = waves * wave_lds + (a.lds_bits ? bits_bytes : 0);
     const int block = waves * kWave;
-    void (*kern)(Args) = L.cache ? (a.lds_bits ? propagate_persistent<true, true> : propagate_persistent<true, false>)
-                                 : (a.lds_bits ? propagate_persistent<false, true> : propagate_persistent<false, false>);
+    void (*kern)(Args) = persistent_kernel(tw, L.cache != 0, a.lds_bits != 0);
     int per_cu = 0;
     PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, shmem));
     if (per_cu < 1) return set_error(PG_ERR_INVALID, "persistent kernel does not fit on a CU");
@@ -2328,13 +2396,15 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
                                 const float *az, const float *ecc, float *time, uint16_t *refs_x, uint16_t *refs_y,
                                 uint32_t *boundaries, const pg_tuning *tuning, pg_stats *stats, hipStream_t stream,
                                 const CallSpace *space) {
+  pg_tuning tn{};
+  if (tuning) tn = *tuning;
+  if ((tn.tile_w || tn.tile_h) && !tile_shape_ok(tn.tile_w, tn.tile_h))
+    return set_error(PG_ERR_INVALID, "tile shape must be 8x8, 16x4 or 32x2");
   Scen sc;
-  pg_status st = scen_geometry(settings, &sc);
+  pg_status st = scen_geometry(settings, &sc, tn.tile_w ? static_cast<int>(tn.tile_w) : kTile);
   if (st != PG_OK) return st;
   if (!speed || !az || !ecc || !time || !refs_x || !refs_y || !boundaries)
     return set_error(PG_ERR_INVALID, "null raster pointer");
-  pg_tuning tn{};
-  if (tuning) tn = *tuning;
   if (tn.max_iterations) sc.max_iters = tn.max_iterations;
   if (sc.max_iters > 0x3fffffffu) return set_error(PG_ERR_INVALID, "max_iterations too large");
 
@@ -2426,6 +2496,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   L.k_begin = 0;
   L.k_end = static_cast<int>(sc.max_iters) + 1;  // the last sweep only detects a cap
   L.cache = sc.cache;
+  L.tw = sc.tw;
   L.lds_bits_src = boundaries;
   L.lds_bits_words = (W * H + 31) / 32;
   L.ws = space ? space->launch : nullptr;

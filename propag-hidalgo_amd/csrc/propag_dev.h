@@ -12,7 +12,23 @@
 
 namespace pg {
 
-constexpr int kTile = 8;  // tile edge (cells): one cell per lane of a wave
+constexpr int kTile = 8;  // default tile edge (cells): 8 x 8, one cell per lane of a wave
+
+// Tile shapes: tw x (64 / tw) cells, one cell per lane of a wave.
+inline bool tile_shape_ok(uint32_t w, uint32_t h) {
+  return (w == 8 && h == 8) || (w == 16 && h == 4) || (w == 32 && h == 2);
+}
+inline size_t tiles_of(size_t W, size_t H, int tw) {
+  const size_t th = static_cast<size_t>(64 / tw);
+  return ((W + tw - 1) / tw) * ((H + th - 1) / th);
+}
+// The most tiles any shape cuts a raster into (buffers sized for every shape).
+inline size_t max_tiles(size_t W, size_t H) {
+  size_t m = tiles_of(W, H, 8);
+  if (tiles_of(W, H, 16) > m) m = tiles_of(W, H, 16);
+  if (tiles_of(W, H, 32) > m) m = tiles_of(W, H, 32);
+  return m;
+}
 
 // Bytes of the device array behind pg_propagate_noalloc: the call and launch
 // workspaces of any raster of at most PG_NOALLOC_MAX_CELLS cells (the largest,
@@ -42,9 +58,11 @@ struct Scen {
   const uint32_t *bits;  // boundary bitset (row-major, the output layout)
   const unsigned long long *blk;  // the same bits in 8x8 blocks (walks)
   uint4 *box;            // per tile: bounding box (x0, y0, x1, y1, in cells) of its far dependents
-  unsigned long long *isref;  // per tile: its cells that are some cell's far reference (8x8 bits)
+  unsigned long long *isref;  // per tile: its cells that are some cell's far reference (bit x + tw y)
   uint32_t *mark[2];     // tiles queued for a sweep of parity 0 / 1
-  int ntx, nty;          // tiles (= 8x8 bit blocks) per row / column
+  int ntx, nty;          // tiles per row / column
+  int tw, th;            // tile shape (cells)
+  int bw, bh;            // 8x8 blocks of the blocked bitset per row / column
   uint32_t nwords_log2, pad0;
   int k0;                // sweep number of the scenario's first sweep
   unsigned max_iters;    // sweeps allowed from k0
@@ -99,9 +117,9 @@ struct ScenBuffers {
 size_t scen_buffer_bytes(size_t width, size_t height);
 ScenBuffers scen_carve(void *base, size_t width, size_t height);
 
-// Fills sc's geometry and tiling from a Settings (buffers and bitmap range
-// are the caller's).
-pg_status scen_geometry(const Settings *settings, Scen *sc);
+// Fills sc's geometry and its tiling into tiles of tw x (64 / tw) cells from
+// a Settings (buffers and bitmap range are the caller's).
+pg_status scen_geometry(const Settings *settings, Scen *sc, int tw = kTile);
 
 // Boundaries, coefficients, initial state and first-sweep tiles of scenario
 // number si (start sweep sc.k0): writes `boundaries`, the scenario's
@@ -140,6 +158,7 @@ struct Launch {
   TileQueue q;
   int k_begin, k_end;
   int cache;           // all scenarios cache line-of-sight flags
+  int tw;              // tile width of all scenarios (their Scen::tw)
   const uint32_t *lds_bits_src;  // nscen == 1: the (row-major) bitset, staged in LDS if it fits
   size_t lds_bits_words;
   char *ws;            // launch workspace of launch_ws_bytes(cu count) bytes, or null for the

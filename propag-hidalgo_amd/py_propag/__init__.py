@@ -665,6 +665,13 @@ class Context:
         except Exception:
             pass
 
+    def set_tile(self, tile_w: int, tile_h: int) -> None:
+        """Tile shape (8x8 default, 16x4 or 32x2) of the scenarios admitted
+        from now on; refused (PG_ERR_BUSY) while any scenario is admitted.
+        Results do not depend on it; ``run``'s tuning may name it or leave
+        ``tile_w``/``tile_h`` out."""
+        _raise_for(self._lib.pg_ctx_set_tile(self._h, int(tile_w), int(tile_h)), "pg_ctx_set_tile")
+
     def add_terrain(self, geo_ref: GeoReference, fuel, slope, aspect) -> int:
         n = len(geo_ref)
         fuel = np.ascontiguousarray(np.asarray(fuel, np.uint8).ravel())

@@ -18,7 +18,8 @@ EXPORTS = ("pg_last_error", "pg_version", "pg_catalog_f32", "pg_burn", "pg_propa
            "pg_scenario_bytes", "pg_scenario_admit", "pg_batch_run", "pg_scenario_status",
            "pg_scenario_result", "pg_scenario_release", "pg_ctx_take_finished", "pg_moisture",
            "pg_rasterize_times", "pg_rasterize_fuels", "pg_init_refs", "pg_write_times_geotiff",
-           "pg_write_refs_gpkg", "pg_write_tiles_gpkg", "pg_propagate_noalloc", "pg_shutdown", "pg_transform_points")
+           "pg_write_refs_gpkg", "pg_write_tiles_gpkg", "pg_propagate_noalloc", "pg_shutdown", "pg_transform_points",
+           "pg_ctx_set_tile")
 
 
 class GT(C.Structure):
@@ -136,6 +137,7 @@ def load(require_torch: bool = True):
     i32, u32 = C.c_int, C.c_uint32
     for name, args in (("pg_ctx_create", [sz, C.POINTER(vp)]),
                        ("pg_ctx_destroy", [vp]),
+                       ("pg_ctx_set_tile", [vp, u32, u32]),
                        ("pg_terrain_add", [vp, C.POINTER(GeoReferenceC), vp, vp, vp, C.POINTER(i32)]),
                        ("pg_scenario_admit", [vp, i32, C.POINTER(Weather), vp, sz, C.c_float, u32,
                                               C.POINTER(i32)]),
