@@ -105,10 +105,12 @@ def autotuned(args, ctx, tid, cases, W, H, tuning):
     """The ensemble's launch tuning: the autotuner's pick on a sample of
     --tune-sample scenarios (or the fixed bench tuning with --no-autotune /
     --profile-phases)."""
-    if args.tuning:
-        w, g, m = (int(v) for v in args.tuning.split(","))
-        return {**tuning, "waves_per_group": w, "workgroups": g, "mode": m}, None
-    if args.no_autotune or args.profile_phases or args.mode == 1:
+    if args.tuning or args.no_autotune or args.profile_phases or args.mode == 1:
+        if tuning.get("tile_w"):
+            ctx.set_tile(tuning["tile_w"], tuning["tile_h"])
+        if args.tuning:
+            w, g, m = (int(v) for v in args.tuning.split(","))
+            return {**tuning, "waves_per_group": w, "workgroups": g, "mode": m}, None
         return dict(tuning), None
     from py_propag.autotune import Autotuner
 
@@ -118,7 +120,8 @@ def autotuned(args, ctx, tid, cases, W, H, tuning):
     table = tuner.tables[tuner.key(W, H, len(cases))]
     log(f"autotune ({len(table)} candidates on {min(len(cases), args.tune_sample)} scenarios, "
         f"{time.perf_counter() - t0:.1f}s): best {best}; "
-        + ", ".join(f"{t['waves_per_group']}w/{t['workgroups']}g/m{t['mode']} {dt * 1e3:.1f}ms" for t, dt in table))
+        + ", ".join(f"{t['tile_w']}x{t['tile_h']}/{t['waves_per_group']}w/{t['workgroups']}g/m{t['mode']} "
+                    f"{dt * 1e3:.1f}ms" for t, dt in table))
     return dict(best), {"best": best, "sample": min(len(cases), args.tune_sample),
                         "table": [{"tuning": t, "ms": dt * 1e3} for t, dt in table]}
 
@@ -283,6 +286,8 @@ def serve_stream(args, pp, synth, torch, tuning):
         return [Request(tids[c["terrain"]], c["wx"], c["ign"], c["max_time"], 0.0 if warm else c["arrival"],
                         tag=(-1 if warm else i, c["terrain"])) for i, c in enumerate(cases)]
 
+    if tuning.get("tile_w"):
+        ctx.set_tile(tuning["tile_w"], tuning["tile_h"])
     server = Server(ctx, on_done=on_done, tuning=tuning)
     server.play(requests(wl["requests"][:8], warm=True))        # untimed: pools, code paths
     server.launches, server.launch_ms = 0, 0.0
@@ -455,6 +460,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="run the ensembles with the default tuning")
     ap.add_argument("--tune-sample", type=int, default=32, help="scenarios per autotuner trial")
     ap.add_argument("--tuning", default="", help="fixed ensemble tuning WAVES,GROUPS,MODE instead of the autotuner")
+    ap.add_argument("--tile", default="", help="tile shape WxH (8x8, 16x4, 32x2) of the fixed tunings and the "
+                                               "one-per-call runs (the autotuner searches all three)")
     ap.add_argument("--profile-phases", action="store_true",
                     help="time the sweep phases inside the kernel (slower; diagnostics only)")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -485,6 +492,9 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     tuning = {"mode": args.mode, "profile": 1 if args.profile_phases else 0}
+    if args.tile:
+        tw, th = (int(v) for v in args.tile.lower().split("x"))
+        tuning.update({"tile_w": tw, "tile_h": th})
 
     # ---------------------------------------------------------- workload
     t0 = time.time()

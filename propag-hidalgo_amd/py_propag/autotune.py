@@ -1,17 +1,18 @@
 """Autotuner of the batched propagation launch.
 
-It picks the persistent kernel's super-grid — workgroups per launch and
-waves per workgroup, i.e. how many 8x8 tiles are in flight per CU and how
+It picks the tile shape (8x8, 16x4 or 32x2 cells, each one cell per lane of
+a 64-lane wave), the persistent kernel's super-grid — workgroups per launch
+and waves per workgroup, i.e. how many tiles are in flight per CU and how
 the per-sweep tile queue is split — and the launch form (direct cooperative
 launch or the cached hipGraph), by timing candidates on a sample of the
-ensemble to be run.  The sweep schedule is exact Jacobi, so results never
-depend on the tuning and any sample is a fair trial; the tuner checks that
-every trial finishes every scenario in the same number of sweeps.
+ensemble to be run, each ``repeats`` times (the median counts).  The sweep
+schedule is exact Jacobi, so results never depend on the tuning and any
+sample is a fair trial; the tuner checks that every trial finishes every
+scenario in the same number of sweeps.
 
-Two shapes are fixed rather than tuned: the tile (8x8 cells, one per lane
-of a 64-lane wave) and one sweep per grid barrier (more local sweeps
-between barriers would change the arrival times the exact schedule
-defines).
+One shape is fixed rather than tuned: one sweep per grid barrier (more
+local sweeps between barriers would change the arrival times the exact
+schedule defines).
 
 The reference sizes its super-grid from the occupancy of its cooperative
 kernel (crates/propag/src/propag.rs:403-418) and has no tuner.
@@ -26,12 +27,15 @@ import numpy as np
 from . import PropagCudaError
 
 
+TILE_SHAPES = ((8, 8), (16, 4), (32, 2))
+
+
 def default_space():
-    """Candidate tunings: waves per workgroup x fraction of the workgroups
-    that fit x launch form."""
+    """Candidate tunings: tile shape x waves per workgroup x fraction of the
+    workgroups that fit x launch form."""
     out = []
-    for waves, frac, mode in itertools.product((16, 8, 4), (1.0, 0.5), (0, 2)):
-        out.append({"waves_per_group": waves, "group_fraction": frac, "mode": mode})
+    for (tw, th), waves, frac, mode in itertools.product(TILE_SHAPES, (16, 8, 4), (1.0, 0.5), (0, 2)):
+        out.append({"tile_w": tw, "tile_h": th, "waves_per_group": waves, "group_fraction": frac, "mode": mode})
     return out
 
 
@@ -57,10 +61,13 @@ class Autotuner:
         return (p2(width), p2(height), p2(count))
 
     def _trial(self, terrain_id, cases, tuning):
-        """Runs the sample once with ``tuning``; the admitted scenarios are
+        """Runs the sample once with ``tuning`` (in its tile shape, set on
+        the context while it holds no scenario); the admitted scenarios are
         released whatever happens.  A launch error propagates (the
         scenarios are then FAILED and released here)."""
         ids = []
+        if tuning is not None and tuning.get("tile_w"):
+            self.ctx.set_tile(tuning["tile_w"], tuning["tile_h"])
         try:
             for c in cases:
                 i = self.ctx.admit(terrain_id, c["wx"], c["ign"], c["max_time"])
@@ -80,9 +87,11 @@ class Autotuner:
 
     def tune(self, terrain_id: int, cases, width: int, height: int, count: int | None = None):
         """Runs every candidate on ``cases`` (a sample of the ensemble) and
-        returns the fastest tuning (a dict for Context.run)."""
+        returns the fastest tuning (a dict for Context.run).  The context
+        must hold no scenario; it is left in the best tuning's tile shape."""
         k = self.key(width, height, count if count is not None else len(cases))
         if k in self.best:
+            self._set_tile(self.best[k])
             return self.best[k]
         # reference sweep counts from an untuned run; the device's full
         # workgroup count for each waves-per-group choice is what a launch
@@ -92,10 +101,12 @@ class Autotuner:
         table = []
         for cand in sorted(self.space, key=lambda c: -c["group_fraction"]):
             w = cand["waves_per_group"]
-            if cand["group_fraction"] < 1.0 and w not in full:
-                continue  # needs the full count of this shape first
-            groups = 0 if cand["group_fraction"] >= 1.0 else max(1, int(full[w] * cand["group_fraction"]))
-            tuning = {"waves_per_group": w, "mode": cand["mode"], "workgroups": groups}
+            tile = (cand.get("tile_w", 8), cand.get("tile_h", 8))
+            if cand["group_fraction"] < 1.0 and (w, tile) not in full:
+                continue  # needs the full count of this launch shape first
+            groups = 0 if cand["group_fraction"] >= 1.0 else max(1, int(full[(w, tile)] * cand["group_fraction"]))
+            tuning = {"tile_w": tile[0], "tile_h": tile[1], "waves_per_group": w, "mode": cand["mode"],
+                      "workgroups": groups}
             times = []
             try:
                 for _ in range(self.repeats):
@@ -108,9 +119,13 @@ class Autotuner:
                 # the candidate; its sample was released
                 self.rejected.append((tuning, str(e)))
                 continue
-            full.setdefault(w, st["workgroups"])
+            full.setdefault((w, tile), st["workgroups"])
             table.append(({**tuning, "workgroups": st["workgroups"]}, float(np.median(times))))
         table.sort(key=lambda e: e[1])
         self.tables[k] = table
         self.best[k] = table[0][0]
+        self._set_tile(table[0][0])
         return table[0][0]
+
+    def _set_tile(self, tuning):
+        self.ctx.set_tile(tuning.get("tile_w", 8), tuning.get("tile_h", 8))

@@ -316,7 +316,9 @@ def test_autotuner_picks_a_tuning_without_changing_results():
         tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
         tuner = Autotuner(ctx)
         best = tuner.tune(tid, cases, w, h)
-        assert set(best) == {"waves_per_group", "workgroups", "mode"}
+        assert set(best) == {"tile_w", "tile_h", "waves_per_group", "workgroups", "mode"}
+        assert {(t["tile_w"], t["tile_h"]) for t, _ in tuner.tables[tuner.key(w, h, len(cases))]} == \
+            {(8, 8), (16, 4), (32, 2)}
         assert len(tuner.tables[tuner.key(w, h, len(cases))]) == len(tuner.space)
         assert tuner.tune(tid, cases, w, h) is best   # cached per workload class
         ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]

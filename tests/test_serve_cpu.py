@@ -153,17 +153,28 @@ def test_server_raises_when_a_request_can_never_fit():
 
 
 class TunerContext(FakeContext):
-    """Trials cost what the tuning says: mode 2 with 8 waves is fastest."""
+    """Trials cost what the tuning says: 16x4 tiles, mode 2 with 8 waves is
+    fastest.  Like the real context it takes a tile shape only while it
+    holds no scenario, and a tuning must name the shape it holds."""
 
     def __init__(self):
         super().__init__(slots=1000)
         self.clock = Clock()
+        self.tile = (8, 8)
+
+    def set_tile(self, tw, th):
+        if any(s is not None for s in self.state.values()):
+            raise RuntimeError("set_tile with scenarios admitted")
+        self.tile = (tw, th)
 
     def run(self, max_sweeps=0, tuning=None):
-        fin, st = super().run(max_sweeps, tuning)
         t = dict(tuning or {})
+        if t.get("tile_w") and (t["tile_w"], t["tile_h"]) != self.tile:
+            raise RuntimeError("tuning names another tile shape than the context's")
+        fin, st = super().run(max_sweeps, tuning)
         w = t.get("waves_per_group", 16) or 16
         cost = {16: 3.0, 8: 1.0, 4: 2.0}[w] + (0.0 if t.get("mode", 0) == 2 else 0.5)
+        cost += {(8, 8): 0.2, (16, 4): 0.0, (32, 2): 0.4}[self.tile]
         cost *= 1.0 if not t.get("workgroups") or t["workgroups"] >= 4096 // w else 1.7
         self.clock.t += cost
         st["workgroups"] = t.get("workgroups") or 4096 // w
@@ -175,7 +186,8 @@ def test_autotuner_picks_the_fastest_and_caches_per_class():
     cases = [{"wx": {"work": 5 + i}, "ign": [(1, 1, 0.0)], "max_time": 3600.0} for i in range(6)]
     tuner = Autotuner(ctx, clock=ctx.clock)
     best = tuner.tune(0, cases, 1000, 1000, count=128)
-    assert best == {"waves_per_group": 8, "mode": 2, "workgroups": 512}
+    assert best == {"tile_w": 16, "tile_h": 4, "waves_per_group": 8, "mode": 2, "workgroups": 512}
+    assert ctx.tile == (16, 4)      # the context is left in the best shape
     table = tuner.tables[tuner.key(1000, 1000, 128)]
     assert len(table) == len(default_space())
     assert [dt for _, dt in table] == sorted(dt for _, dt in table)
