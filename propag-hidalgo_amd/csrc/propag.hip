@@ -473,7 +473,7 @@ constexpr int kPairsMax = kWave * 8;  // (cell, neighbour) pairs of one tile
 
 template <int TW>
 struct TileLds {  // one wave's staged tile (TW x 64 / TW cells + 1-cell halo)
-  static constexpr int kCells = (Tile<TW>::Cells + 7) / 8 * 8;
+  static constexpr int kCells = Tile<TW>::Cells;
   unsigned long long key[kWave];  // per cell: least candidate key (packed rule)
   float time[kCells];
   uint32_t ref[kCells];
