@@ -402,41 +402,48 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
   };
   st = grow_queue(c, live_tiles(c) + s.tiles);
   if (st != PG_OK) return fail(st);
-  // Per-cell wind, if given, goes through the time / refs buffers (8 bytes
-  // per cell) before the initial state overwrites them.
-  const float *wsf = nullptr, *wazf = nullptr;
-  if (w->wind_speed_field || w->wind_azimuth_field) {
-    if (!w->wind_speed_field || !w->wind_azimuth_field)
-      return fail(set_error(PG_ERR_INVALID, "wind fields come in pairs"));
-    e = hipMemcpyAsync(s.time, w->wind_speed_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(s.rx, w->wind_azimuth_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return fail(hip_error(e, "wind upload"));
-    wsf = s.time;
-    wazf = reinterpret_cast<const float *>(s.rx);
-  }
   const float moist[5] = {w->d1hr, w->d10hr, w->d100hr, w->herb, w->wood};
-  st = launch_burn_weather(c->dev, n, t.fuel, t.slope, t.aspect, moist, w->wind_speed, w->wind_azimuth, wsf, wazf,
-                           s.speed, s.az, s.ecc, c->stream);
-  if (st != PG_OK) return fail(st);
-  pg_ignition *d_ign = nullptr;
   const size_t n_uniq = uniq.size();
-  if (n_uniq) {
-    e = hipMalloc(reinterpret_cast<void **>(&d_ign), n_uniq * sizeof(pg_ignition));
-    if (e != hipSuccess) return fail(hip_error(e, "ignition buffer"));
-    c->dev_ign.push_back(d_ign);
-    c->host_ign.push_back(std::move(uniq));
-    const std::vector<pg_ignition> &hv = c->host_ign.back();
-    e = hipMemcpyAsync(d_ign, hv.data(), n_uniq * sizeof(pg_ignition), hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return fail(hip_error(e, "ignition upload"));
+  const bool fields = w->wind_speed_field || w->wind_azimuth_field;
+  if (fields && (!w->wind_speed_field || !w->wind_azimuth_field))
+    return fail(set_error(PG_ERR_INVALID, "wind fields come in pairs"));
+  const float *wsf = nullptr, *wazf = nullptr;
+  if (!fields && n_uniq <= kAdmitIgnitions) {
+    // constant weather, few ignitions: two launches, ignitions by value
+    st = scen_admit(c->dev, sc, static_cast<uint32_t>(slot), c->d_out + slot, c->q, t.fuel, t.slope, t.aspect, moist,
+                    w->wind_speed, w->wind_azimuth, s.speed, s.az, s.ecc, s.time, s.rx, s.ry, s.bits, uniq.data(),
+                    n_uniq, c->stream);
+    if (st != PG_OK) return fail(st);
+  } else {
+    // Per-cell wind goes through the time / refs buffers (8 bytes per cell)
+    // before the initial state overwrites them.
+    if (fields) {
+      e = hipMemcpyAsync(s.time, w->wind_speed_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(s.rx, w->wind_azimuth_field, n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return fail(hip_error(e, "wind upload"));
+      wsf = s.time;
+      wazf = reinterpret_cast<const float *>(s.rx);
+    }
+    st = launch_burn_weather(c->dev, n, t.fuel, t.slope, t.aspect, moist, w->wind_speed, w->wind_azimuth, wsf, wazf,
+                             s.speed, s.az, s.ecc, c->stream);
+    if (st != PG_OK) return fail(st);
+    pg_ignition *d_ign = nullptr;
+    if (n_uniq) {
+      e = hipMalloc(reinterpret_cast<void **>(&d_ign), n_uniq * sizeof(pg_ignition));
+      if (e != hipSuccess) return fail(hip_error(e, "ignition buffer"));
+      c->dev_ign.push_back(d_ign);
+      c->host_ign.push_back(std::move(uniq));
+      const std::vector<pg_ignition> &hv = c->host_ign.back();
+      e = hipMemcpyAsync(d_ign, hv.data(), n_uniq * sizeof(pg_ignition), hipMemcpyHostToDevice, c->stream);
+      if (e != hipSuccess) return fail(hip_error(e, "ignition upload"));
+    }
+    st = scen_init_state(s.time, s.rx, s.ry, n, W, d_ign, n_uniq, c->stream);
+    if (st != PG_OK) return fail(st);
+    st = scen_prepare(c->dev, sc, static_cast<uint32_t>(slot), c->d_out + slot, c->q, s.speed, s.az, s.ecc, s.time,
+                      s.rx, s.ry, s.bits, false, c->stream);
+    if (st != PG_OK) return fail(st);
   }
-  st = scen_init_state(s.time, s.rx, s.ry, n, W, d_ign, n_uniq, c->stream);
-  if (st != PG_OK) return fail(st);
-  st = scen_fill_u32(sc.mark[0], 2 * static_cast<size_t>(nwords), 0u, c->stream);
-  if (st != PG_OK) return fail(st);
-  st = scen_prepare(c->dev, sc, static_cast<uint32_t>(slot), c->d_out + slot, c->q, s.speed, s.az, s.ecc, s.time,
-                    s.rx, s.ry, s.bits, false, c->stream);
-  if (st != PG_OK) return fail(st);
   s.used = true;
   c->scen[slot] = s;  // stable storage (reserved vector) for the async descriptor copy
   st = put_descriptor(c, slot, &c->scen[slot].sc);

@@ -48,6 +48,7 @@
 #include <mutex>
 #include <vector>
 
+#include "firelib_dev.h"
 #include "pg_internal.h"
 #include "propag_dev.h"
 
@@ -1601,17 +1602,16 @@ __global__ void fill_u32(uint32_t *p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
-__global__ void init_boxes(uint4 *box, unsigned long long *isref, size_t n) {
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    box[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
-    isref[i] = 0ull;
-  }
-}
-
-__global__ void init_out(ScenOut *o, int k0) {
-  o->last_change = k0 - 1;
-  o->capped = 0u;
+// time_from's per-cell coefficients of a fire (speed, azimuth, eccentricity).
+__device__ inline Coef make_coef(float s, float az, float e) {
+  double sn, cs;
+  sincos_det(static_cast<double>(az), sn, cs);
+  Coef k;
+  k.a = static_cast<float>(static_cast<double>(e) * cs);
+  k.b = static_cast<float>(static_cast<double>(e) * sn);
+  k.r = static_cast<float>(1.0 / (static_cast<double>(s) * (1.0 - static_cast<double>(e))));
+  k.null_fire = s == 0.0f ? 1.0f : 0.0f;
+  return k;
 }
 
 // Fire coefficients, packed double-buffered state, stamps, reference
@@ -1625,15 +1625,7 @@ __global__ void prep(Scen sc, uint32_t si, TileQueue tq, unsigned *err, const fl
   const size_t n = static_cast<size_t>(g.W) * g.H;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const float s = speed[i], e = ecc[i];
-    double sn, cs;
-    sincos_det(static_cast<double>(az[i]), sn, cs);
-    Coef k;
-    k.a = static_cast<float>(static_cast<double>(e) * cs);
-    k.b = static_cast<float>(static_cast<double>(e) * sn);
-    k.r = static_cast<float>(1.0 / (static_cast<double>(s) * (1.0 - static_cast<double>(e))));
-    k.null_fire = s == 0.0f ? 1.0f : 0.0f;
-    const_cast<Coef *>(sc.coef)[i] = k;
+    const_cast<Coef *>(sc.coef)[i] = make_coef(speed[i], az[i], ecc[i]);
     const float t = time[i];
     uint32_t ref = static_cast<uint32_t>(rx[i]) | (static_cast<uint32_t>(ry[i]) << 16);
     const bool burned = t < FLT_MAX;
@@ -1692,6 +1684,31 @@ __global__ void block_bits(const uint32_t *bits, unsigned long long *blk, uint32
   }
 }
 
+// find_boundaries' predicate for cell (x, y) (propag.cu:345-375,296-302):
+// an in-bounds neighbour has a dissimilar fire that is faster.  The
+// similarity thresholds are compared in double, as the reference's f64
+// constants are.
+__device__ inline bool boundary_at(const Geom &g, const float *__restrict__ speed, const float *__restrict__ az,
+                                   const float *__restrict__ ecc, int x, int y) {
+  const size_t i = static_cast<size_t>(x) + static_cast<size_t>(y) * g.W;
+  const float s0 = speed[i], a0 = az[i], e0 = ecc[i];
+  bool changed = false;
+  for (int di = -1; di < 2; ++di) {
+    for (int dj = -1; dj < 2; ++dj) {
+      if (di == 0 && dj == 0) continue;
+      const int nx = x + di, ny = y + dj;
+      if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H) continue;
+      const size_t j = static_cast<size_t>(nx) + static_cast<size_t>(ny) * g.W;
+      const float s1 = speed[j], a1 = az[j], e1 = ecc[j];
+      const bool similar = static_cast<double>(fabsf(s1 - s0)) < 0.1 &&
+                           static_cast<double>(fabsf(a1 - a0)) < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
+                           static_cast<double>(fabsf(e1 - e0)) < 0.05;
+      changed = changed || (!similar && s0 < s1);
+    }
+  }
+  return changed;
+}
+
 __global__ __launch_bounds__(256) void boundaries_kernel(Geom g, const float *speed, const float *az,
                                                         const float *ecc, uint32_t *bits) {
   const size_t n = static_cast<size_t>(g.W) * g.H;
@@ -1699,29 +1716,176 @@ __global__ __launch_bounds__(256) void boundaries_kernel(Geom g, const float *sp
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t base = static_cast<size_t>(blockIdx.x) * blockDim.x; base < nwords * 32; base += stride) {
     const size_t i = base + threadIdx.x;
-    bool changed = false;
-    if (i < n) {
-      const int x = static_cast<int>(i % g.W), y = static_cast<int>(i / g.W);
-      const float s0 = speed[i], a0 = az[i], e0 = ecc[i];
-      for (int di = -1; di < 2; ++di) {
-        for (int dj = -1; dj < 2; ++dj) {
-          if (di == 0 && dj == 0) continue;
-          const int nx = x + di, ny = y + dj;
-          if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H) continue;
-          const size_t j = static_cast<size_t>(nx) + static_cast<size_t>(ny) * g.W;
-          const float s1 = speed[j], a1 = az[j], e1 = ecc[j];
-          const bool similar = static_cast<double>(fabsf(s1 - s0)) < 0.1 &&
-                               static_cast<double>(fabsf(a1 - a0)) < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
-                               static_cast<double>(fabsf(e1 - e0)) < 0.05;
-          changed = changed || (!similar && s0 < s1);
-        }
-      }
-    }
+    const bool changed =
+        i < n && boundary_at(g, speed, az, ecc, static_cast<int>(i % g.W), static_cast<int>(i / g.W));
     const unsigned long long m = __ballot(changed);
     const int lane = threadIdx.x & 63;
     const size_t w0 = (base + (threadIdx.x & ~63u)) / 32;
     if (lane == 0 && w0 < nwords) bits[w0] = static_cast<uint32_t>(m);
     if (lane == 32 && w0 + 1 < nwords) bits[w0 + 1] = static_cast<uint32_t>(m >> 32);
+  }
+}
+
+// Point ignitions of an admitted scenario, passed by value (at most
+// kInlineIgn distinct cells; more go through init_state / set_ignitions).
+constexpr int kInlineIgn = static_cast<int>(kAdmitIgnitions);
+struct IgnList {
+  uint32_t n;
+  uint32_t xy[kInlineIgn];  // x | y << 16
+  float t[kInlineIgn];
+};
+
+// The initial state of the ignited cells of a scenario whose other cells
+// admit_cells has set unburned (propag.rs:433-449 for points): each takes
+// its time and itself as reference (never walked: an ignition's own
+// reference), its stamp marks it changed before the first sweep, and the
+// 3x3 tile neighbourhood of its tile is queued for that sweep.  Thread t of
+// one workgroup handles ignition t / 9, neighbour tile t % 9.
+__device__ inline void seed_ignitions(const Scen &sc, uint32_t si, const TileQueue &q, const IgnList &ign,
+                                      float *time, uint16_t *rx, uint16_t *ry) {
+  for (unsigned t = threadIdx.x; t < 9u * ign.n; t += blockDim.x) {
+    const unsigned e = t / 9u, d = t % 9u;
+    const int x = static_cast<int>(ign.xy[e] & 0xffffu), y = static_cast<int>(ign.xy[e] >> 16);
+    if (d == 4u) {
+      const size_t i = static_cast<size_t>(x) + static_cast<size_t>(y) * sc.g.W;
+      const uint2 w = pack_state(ign.t[e], make_ref(x, y));
+      sc.st[0][i] = w;
+      sc.st[1][i] = w;
+      sc.stamp[i] = sc.k0 - 1;
+      time[i] = ign.t[e];
+      rx[i] = static_cast<uint16_t>(x);
+      ry[i] = static_cast<uint16_t>(y);
+    }
+    const int X = x / sc.tw + static_cast<int>(d % 3u) - 1, Y = y / sc.th + static_cast<int>(d / 3u) - 1;
+    if (X >= 0 && Y >= 0 && X < sc.ntx && Y < sc.nty) queue_tile(q, sc, si, sc.k0, X, Y);
+  }
+}
+
+// Seeding of the ignitions by itself (the bitset pass below does it on
+// rasters whose width is a multiple of 32).
+struct Seed {
+  Scen sc;
+  TileQueue q;
+  IgnList ign;
+  uint32_t si;
+  float *time;
+  uint16_t *rx, *ry;
+};
+__global__ void seed_kernel(Seed a) { seed_ignitions(a.sc, a.si, a.q, a.ign, a.time, a.rx, a.ry); }
+
+// boundaries_kernel and block_bits in one pass, for rasters whose width is a
+// multiple of 32: a workgroup covers 32 x 8 cells (four 8x8 blocks), wave w
+// its rows 2w and 2w + 1, so each row of 32 cells is exactly one word of the
+// row-major bitset and one byte of each of the four blocked words.  With
+// `seed`, workgroup (0, 0) also seeds the admitted scenario's ignitions.
+__global__ __launch_bounds__(256) void bits_fused(Geom g, const float *speed, const float *az, const float *ecc,
+                                                 uint32_t *bits, unsigned long long *blk, uint32_t bw, int seed,
+                                                 Seed sd) {
+  __shared__ uint32_t rows[8];
+  const int c = static_cast<int>(threadIdx.x & 31u), r = static_cast<int>(threadIdx.x >> 5);
+  const int x0 = static_cast<int>(blockIdx.x) * 32, y0 = static_cast<int>(blockIdx.y) * 8;
+  const int y = y0 + r;
+  const bool b = y < g.H && boundary_at(g, speed, az, ecc, x0 + c, y);
+  const unsigned long long m = __ballot(b);
+  const unsigned lane = threadIdx.x & 63u;
+  if (lane == 0u || lane == 32u) {
+    const uint32_t word = static_cast<uint32_t>(lane == 0u ? m : m >> 32);
+    rows[r] = word;
+    if (y < g.H) bits[(static_cast<size_t>(x0) + static_cast<size_t>(y) * g.W) >> 5] = word;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4u) {
+    const unsigned j = threadIdx.x;
+    unsigned long long v = 0;
+#pragma unroll
+    for (int rr = 0; rr < 8; ++rr) v |= static_cast<unsigned long long>((rows[rr] >> (8u * j)) & 0xffu) << (8 * rr);
+    blk[static_cast<size_t>(x0 / 8 + static_cast<int>(j)) + static_cast<size_t>(blockIdx.y) * bw] = v;
+  }
+  if (seed && blockIdx.x == 0 && blockIdx.y == 0) seed_ignitions(sd.sc, sd.si, sd.q, sd.ign, sd.time, sd.rx, sd.ry);
+}
+
+// Fills of a call's set-up in one launch: word i of job j is (i & 3) < 2 ?
+// v01 : v23 (a plain fill has v01 == v23; the dependents boxes, x0 y0 x1 y1
+// per tile, are ~0 ~0 0 0), and the scenario's outcome record is reset.
+struct FillJob {
+  uint32_t *p;
+  size_t n;
+  uint32_t v01, v23;
+};
+constexpr int kMaxFillJobs = 8;
+struct FillSet {
+  FillJob job[kMaxFillJobs];
+  int njobs;
+  int k0;
+  ScenOut *out;
+};
+
+__global__ __launch_bounds__(256) void fill_set(FillSet f) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (int j = 0; j < f.njobs; ++j) {
+    uint32_t *p = f.job[j].p;
+    const uint32_t v01 = f.job[j].v01, v23 = f.job[j].v23;
+    for (size_t i = t0; i < f.job[j].n; i += stride) p[i] = (i & 2u) ? v23 : v01;
+  }
+  if (t0 == 0 && f.out) {
+    f.out->last_change = f.k0 - 1;
+    f.out->capped = 0u;
+    f.out->err = 0u;
+  }
+}
+
+// Admission of a scenario on a resident terrain with spatially constant
+// weather, in one pass over its cells: the burn (burn_weather's rule: cells
+// without a catalog fuel get a null fire), time_from's coefficients, every
+// cell unburned (state, stamp, result rasters; seed_ignitions then sets the
+// ignited cells), and the scenario's mark bitmaps, dependents boxes,
+// far-reference bitmaps and outcome record cleared.
+struct AdmitArgs {
+  Scen sc;
+  const uint8_t *fuel;
+  const float *slope, *aspect;
+  float d1, d10, d100, herb, wood, ws, waz;
+  float *speed, *az, *ecc;
+  float *time;
+  uint16_t *rx, *ry;
+  ScenOut *out;
+  const DevCatalog *cat;
+  size_t mark_words, tiles;
+};
+
+__global__ __launch_bounds__(256) void admit_cells(AdmitArgs a) {
+  __shared__ DevFuel s_cat[kNumFuels];
+  firelib::stage_catalog(s_cat, a.cat);
+  const Scen &sc = a.sc;
+  const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  const size_t t0 = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint2 unburned = pack_state(FLT_MAX, kNoRef);
+  for (size_t i = t0; i < n; i += stride) {
+    firelib::Fire f{0.0f, 0.0f, 0.0f};
+    (void)firelib::burn_cell(s_cat, a.fuel[i], a.d1, a.d10, a.d100, a.herb, a.wood, a.ws, a.waz, a.slope[i],
+                             a.aspect[i], f);
+    a.speed[i] = f.speed;
+    a.az[i] = f.az;
+    a.ecc[i] = f.ecc;
+    const_cast<Coef *>(sc.coef)[i] = make_coef(f.speed, f.az, f.ecc);
+    sc.st[0][i] = unburned;
+    sc.st[1][i] = unburned;
+    sc.stamp[i] = kStampNever;
+    a.time[i] = FLT_MAX;
+    a.rx[i] = 0xFFFFu;
+    a.ry[i] = 0xFFFFu;
+  }
+  for (size_t i = t0; i < a.mark_words; i += stride) sc.mark[0][i] = 0u;  // both parities: contiguous
+  for (size_t i = t0; i < a.tiles; i += stride) {
+    sc.box[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u);
+    sc.isref[i] = 0ull;
+  }
+  if (t0 == 0) {
+    a.out->last_change = sc.k0 - 1;
+    a.out->capped = 0u;
+    a.out->err = 0u;
   }
 }
 
@@ -2008,39 +2172,119 @@ pg_status scen_geometry(const Settings *settings, Scen *sc, int tw) {
   return PG_OK;
 }
 
+namespace {
+
+dim3 setup_grid(const DeviceInfo *dev, size_t items, unsigned per_cu) {
+  size_t b = (items + 255) / 256;
+  const size_t cap = static_cast<size_t>(dev->cu_count) * per_cu;
+  if (b > cap) b = cap;
+  return dim3(static_cast<unsigned>(b ? b : 1));
+}
+
+// The boundary bitset of a scenario in both layouts (row-major `bits`,
+// blocked sc.blk), seeding the ignitions of `seed` if given.
+pg_status launch_bits(const DeviceInfo *dev, const Scen &sc, const float *speed, const float *az, const float *ecc,
+                      uint32_t *bits, const Seed *seed, hipStream_t stream) {
+  unsigned long long *blk = const_cast<unsigned long long *>(sc.blk);
+  if (sc.g.W % 32 == 0) {
+    const dim3 grid(static_cast<unsigned>(sc.g.W / 32), static_cast<unsigned>((sc.g.H + 7) / 8));
+    hipLaunchKernelGGL(bits_fused, grid, dim3(256), 0, stream, sc.g, speed, az, ecc, bits, blk,
+                       static_cast<uint32_t>(sc.bw), seed ? 1 : 0, seed ? *seed : Seed{});
+  } else {
+    const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+    hipLaunchKernelGGL(boundaries_kernel, setup_grid(dev, (n + 31) / 32 * 32, 8), dim3(256), 0, stream, sc.g, speed,
+                       az, ecc, bits);
+    hipLaunchKernelGGL(block_bits, setup_grid(dev, static_cast<size_t>(sc.bw) * sc.bh, 8), dim3(256), 0, stream,
+                       bits, blk, sc.g.W, sc.g.H, static_cast<uint32_t>(sc.bw), static_cast<uint32_t>(sc.bh));
+    if (seed) hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(256), 0, stream, *seed);
+  }
+  PG_HIP(hipGetLastError());
+  return PG_OK;
+}
+
+}  // namespace
+
 pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
                        const float *speed, const float *az, const float *ecc, const float *time,
                        const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries, bool validate,
-                       hipStream_t stream) {
+                       hipStream_t stream, bool clear_queue) {
   const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
   const size_t ntiles = static_cast<size_t>(sc.ntx) * sc.nty;
-  const size_t nwords = (n + 31) / 32;
-  const unsigned cu = static_cast<unsigned>(dev->cu_count);
-  auto grid_for = [&](size_t items) {
-    size_t b = (items + 255) / 256;
-    if (b > cu * 8) b = cu * 8;
-    return dim3(static_cast<unsigned>(b ? b : 1));
+  FillSet f{};
+  auto job = [&](void *p, size_t words, uint32_t v01, uint32_t v23) {
+    f.job[f.njobs++] = FillJob{static_cast<uint32_t *>(p), words, v01, v23};
   };
-  unsigned *err = &out->err;
-  hipLaunchKernelGGL(init_boxes, grid_for(ntiles), dim3(256), 0, stream, sc.box, sc.isref, ntiles);
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(64), 0, stream, err, static_cast<size_t>(1), 0u);
-  hipLaunchKernelGGL(boundaries_kernel, grid_for(nwords * 32), dim3(256), 0, stream, sc.g, speed, az, ecc,
-                     boundaries);
-  hipLaunchKernelGGL(block_bits, grid_for(static_cast<size_t>(sc.bw) * sc.bh), dim3(256), 0, stream, boundaries,
-                     const_cast<unsigned long long *>(sc.blk), sc.g.W, sc.g.H, static_cast<uint32_t>(sc.bw),
-                     static_cast<uint32_t>(sc.bh));
-  hipLaunchKernelGGL(prep, grid_for(n), dim3(256), 0, stream, sc, si, q, err, speed, az, ecc, time, refs_x,
-                     refs_y);
+  job(sc.mark[0], 2 * (size_t{1} << sc.nwords_log2), 0u, 0u);  // both parities: contiguous
+  job(sc.box, 4 * ntiles, 0xFFFFFFFFu, 0u);
+  job(sc.isref, 2 * ntiles, 0u, 0u);
+  if (clear_queue) {
+    job(q.list, 8 * static_cast<size_t>(q.nsub) * q.cap, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    job(q.count, 128 * static_cast<size_t>(q.nsub), 0u, 0u);
+  }
+  f.out = out;
+  f.k0 = sc.k0;
+  size_t most = 0;
+  for (int j = 0; j < f.njobs; ++j) most = std::max(most, f.job[j].n);
+  hipLaunchKernelGGL(fill_set, setup_grid(dev, most, 8), dim3(256), 0, stream, f);
+  PG_HIP(hipGetLastError());
+  pg_status st = launch_bits(dev, sc, speed, az, ecc, boundaries, nullptr, stream);
+  if (st != PG_OK) return st;
+  hipLaunchKernelGGL(prep, setup_grid(dev, n, 8), dim3(256), 0, stream, sc, si, q, &out->err, speed, az, ecc, time,
+                     refs_x, refs_y);
   PG_HIP(hipGetLastError());
   unsigned e = 0;
   if (validate) {
-    PG_HIP(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, stream));
+    PG_HIP(hipMemcpyAsync(&e, &out->err, sizeof(e), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
   }
-  hipLaunchKernelGGL(init_out, dim3(1), dim3(1), 0, stream, out, sc.k0);
-  PG_HIP(hipGetLastError());
   if (e & 2u) return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
   return PG_OK;
+}
+
+pg_status scen_admit(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
+                     const uint8_t *fuel, const float *slope, const float *aspect, const float moisture[5],
+                     float wind_speed, float wind_azimuth, float *speed, float *az, float *ecc, float *time,
+                     uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries, const pg_ignition *ign, size_t n_ign,
+                     hipStream_t stream) {
+  if (n_ign > static_cast<size_t>(kInlineIgn)) return set_error(PG_ERR_INVALID, "scen_admit: too many ignitions");
+  const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
+  AdmitArgs a{};
+  a.sc = sc;
+  a.fuel = fuel;
+  a.slope = slope;
+  a.aspect = aspect;
+  a.d1 = moisture[0];
+  a.d10 = moisture[1];
+  a.d100 = moisture[2];
+  a.herb = moisture[3];
+  a.wood = moisture[4];
+  a.ws = wind_speed;
+  a.waz = wind_azimuth;
+  a.speed = speed;
+  a.az = az;
+  a.ecc = ecc;
+  a.time = time;
+  a.rx = refs_x;
+  a.ry = refs_y;
+  a.out = out;
+  a.cat = dev->catalog;
+  a.mark_words = 2 * (size_t{1} << sc.nwords_log2);
+  a.tiles = static_cast<size_t>(sc.ntx) * sc.nty;
+  hipLaunchKernelGGL(admit_cells, setup_grid(dev, n, 16), dim3(256), 0, stream, a);
+  PG_HIP(hipGetLastError());
+  Seed sd{};
+  sd.sc = sc;
+  sd.q = q;
+  sd.si = si;
+  sd.time = time;
+  sd.rx = refs_x;
+  sd.ry = refs_y;
+  sd.ign.n = static_cast<uint32_t>(n_ign);
+  for (size_t i = 0; i < n_ign; ++i) {
+    sd.ign.xy[i] = ign[i].x | (ign[i].y << 16);
+    sd.ign.t[i] = ign[i].time;
+  }
+  return launch_bits(dev, sc, speed, az, ecc, boundaries, &sd, stream);
 }
 
 pg_status scen_init_state(float *time, uint16_t *refs_x, uint16_t *refs_y, size_t n, uint32_t width,
@@ -2465,14 +2709,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
     }
   } guard{{&ev0, &ev1}};
   if (ev0) PG_HIP(hipEventRecord(ev0, stream));
-  const unsigned cu = static_cast<unsigned>(dev->cu_count);
-  const unsigned lgrid = static_cast<unsigned>(std::min<size_t>((8 * nsub * ntiles + 255) / 256, cu * 8));
-  hipLaunchKernelGGL(fill_u32, dim3(cu), dim3(256), 0, stream, sc.mark[0], static_cast<size_t>(2 * nwords), 0u);
-  hipLaunchKernelGGL(fill_u32, dim3(lgrid ? lgrid : 1), dim3(256), 0, stream,
-                     reinterpret_cast<uint32_t *>(q.list), 8 * nsub * ntiles, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(fill_u32, dim3(1), dim3(128), 0, stream, q.count, static_cast<size_t>(128 * nsub), 0u);
-  PG_HIP(hipGetLastError());
-  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, false, stream);
+  st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, false, stream, true);
   if (st != PG_OK) return st;
   PG_HIP(hipMemcpyAsync(d_scen, &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
   // Post-pass and outcome right behind the relaxation, so the call waits on

@@ -123,14 +123,29 @@ pg_status scen_geometry(const Settings *settings, Scen *sc, int tw = kTile);
 
 // Boundaries, coefficients, initial state and first-sweep tiles of scenario
 // number si (start sweep sc.k0): writes `boundaries`, the scenario's
-// rasters, queues in q the tiles of sweep sc.k0 and resets *out.  The
-// scenario's mark bitmaps must be zero.  With `validate`, synchronises the
+// rasters, queues in q the tiles of sweep sc.k0 and resets *out, after
+// clearing the scenario's mark bitmaps, dependents boxes and far-reference
+// bitmaps (and, with `clear_queue`, every list and count of q: a queue of
+// this scenario alone).  With `validate`, synchronises the
 // stream and returns PG_ERR_INVALID if a burned cell has a reference outside
 // the raster (such cells are dropped either way).
 pg_status scen_prepare(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
                        const float *speed, const float *az, const float *ecc, const float *time,
                        const uint16_t *refs_x, const uint16_t *refs_y, uint32_t *boundaries, bool validate,
-                       hipStream_t stream);
+                       hipStream_t stream, bool clear_queue = false);
+
+// Admission of scenario si on a resident terrain (fuel, slope, aspect) with
+// spatially constant moisture and wind and at most kAdmitIgnitions point
+// ignitions (distinct cells): burn pass into speed/az/ecc, boundaries in
+// both layouts, time_from's coefficients, initial state (also into the
+// result rasters time / refs_x / refs_y), the tiles of sweep sc.k0 queued in
+// q and *out reset.  Two launches; nothing else needs to precede it.
+constexpr size_t kAdmitIgnitions = 16;
+pg_status scen_admit(const DeviceInfo *dev, const Scen &sc, uint32_t si, ScenOut *out, const TileQueue &q,
+                     const uint8_t *fuel, const float *slope, const float *aspect, const float moisture[5],
+                     float wind_speed, float wind_azimuth, float *speed, float *az, float *ecc, float *time,
+                     uint16_t *refs_x, uint16_t *refs_y, uint32_t *boundaries, const pg_ignition *ign, size_t n_ign,
+                     hipStream_t stream);
 
 // Initial state from point ignitions (at most one per cell, device array).
 pg_status scen_init_state(float *time, uint16_t *refs_x, uint16_t *refs_y, size_t n, uint32_t width,
