@@ -81,6 +81,10 @@ struct pg_ctx {
   std::vector<pg::Scenario> scen;  // index = scenario id = slot
   pg::Scen *d_scen = nullptr;
   pg::ScenOut *d_out = nullptr;
+  // Host mirror of the descriptors (pinned); slots [desc_lo, desc_hi] changed
+  // since the last upload, which pg_batch_run issues before its launch.
+  pg::Scen *h_scen = nullptr;
+  int desc_lo = 1 << 30, desc_hi = -1;
   pg::TileQueue q{};
   void *q_block = nullptr;
   // Ignition lists of admissions since the last synchronisation (the
@@ -159,10 +163,24 @@ pg_status grow_queue(pg_ctx *c, size_t cap) {
   return PG_OK;
 }
 
-// Copies a descriptor from a host object that stays valid until the next
-// synchronisation of the context's stream.
+// Sets a slot's descriptor in the host mirror; the next launch uploads it.
+// (The mirror changes only while no upload from it is in flight: every
+// pg_batch_run synchronises its stream before returning.)
 pg_status put_descriptor(pg_ctx *c, int slot, const Scen *sc) {
-  PG_HIP(hipMemcpyAsync(c->d_scen + slot, sc, sizeof(Scen), hipMemcpyHostToDevice, c->stream));
+  c->h_scen[slot] = *sc;
+  c->desc_lo = std::min(c->desc_lo, slot);
+  c->desc_hi = std::max(c->desc_hi, slot);
+  return PG_OK;
+}
+
+// Uploads the descriptors changed since the last upload, in one copy.
+pg_status flush_descriptors(pg_ctx *c) {
+  if (c->desc_hi < c->desc_lo) return PG_OK;
+  const size_t n = static_cast<size_t>(c->desc_hi - c->desc_lo + 1);
+  PG_HIP(hipMemcpyAsync(c->d_scen + c->desc_lo, c->h_scen + c->desc_lo, n * sizeof(Scen), hipMemcpyHostToDevice,
+                        c->stream));
+  c->desc_lo = 1 << 30;
+  c->desc_hi = -1;
   return PG_OK;
 }
 
@@ -264,6 +282,7 @@ pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_scen), kMaxSlots * sizeof(Scen));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_out), kMaxSlots * sizeof(ScenOut));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&c->h_scen), kMaxSlots * sizeof(Scen));
   if (e != hipSuccess) {
     st = hip_error(e, "pg_ctx_create");
     pg_ctx_destroy(c);
@@ -288,6 +307,7 @@ pg_status pg_ctx_destroy(pg_ctx *c) {
   if (c->q_block) (void)hipFree(c->q_block);
   if (c->d_scen) (void)hipFree(c->d_scen);
   if (c->d_out) (void)hipFree(c->d_out);
+  if (c->h_scen) (void)hipHostFree(c->h_scen);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PG_OK;
@@ -445,7 +465,7 @@ pg_status pg_scenario_admit(pg_ctx *c, int terrain_id, const pg_weather *w, cons
     if (st != PG_OK) return fail(st);
   }
   s.used = true;
-  c->scen[slot] = s;  // stable storage (reserved vector) for the async descriptor copy
+  c->scen[slot] = s;
   st = put_descriptor(c, slot, &c->scen[slot].sc);
   if (st != PG_OK) {
     c->scen[slot] = Scenario{};
@@ -509,6 +529,8 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
         if (*p) (void)hipEventDestroy(*p);
     }
   } guard{{&ev0, &ev1}};
+  pg_status fs = flush_descriptors(c);
+  if (fs != PG_OK) return fs;
   PG_HIP(hipEventRecord(ev0, c->stream));
   Launch L{};
   L.scen = c->d_scen;
@@ -531,6 +553,8 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   PG_HIP(hipMemcpyAsync(outs.data(), c->d_out, nslots * sizeof(ScenOut), hipMemcpyDeviceToHost, c->stream));
   PG_HIP(hipStreamSynchronize(c->stream));
   uint32_t fin = 0;
+  std::vector<PostItem> posts;
+  size_t max_cells = 0;
   for (int i = 0; i < nslots; ++i) {
     Scenario &s = c->scen[i];
     if (s.zombie) {  // its queued entries were consumed (and skipped) by this launch
@@ -549,12 +573,14 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
       s.sweeps = static_cast<uint64_t>(res.last_sweep - s.sc.k0 + 1);
       continue;
     }
-    st = scen_finish(s.sc, scen_final_buf(s.sc, o), s.speed, s.time, s.rx, s.ry, c->stream);
-    if (st != PG_OK) return st;
+    posts.push_back(PostItem{i, scen_final_buf(s.sc, o), s.speed, s.time, s.rx, s.ry});
+    max_cells = std::max(max_cells, static_cast<size_t>(s.sc.g.W) * s.sc.g.H);
     s.posted = true;
     c->finished.push_back(i);
     ++fin;
   }
+  st = scen_finish_batch(c->dev, c->d_scen, posts.data(), posts.size(), max_cells, c->stream);
+  if (st != PG_OK) return st;
   c->k_next = res.last_sweep + 1;
   PG_HIP(hipEventRecord(ev1, c->stream));
   st = sync_ctx(c);

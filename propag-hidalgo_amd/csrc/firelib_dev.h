@@ -18,6 +18,24 @@ struct Fire {
   float speed, az, ecc;
 };
 
+// x^y for the burn's positive finite bases (wind speed, effective-wind
+// terms): exp2(y * log2 x) with x = m * 2^e split exactly (m in [0.5, 1)),
+// log2 m from the hardware log2 (V_LOG_F32, |log2 m| <= 1, so its error is
+// absolute and at most an ulp of 1), e + log2 m and the product with y
+// formed in double, 2^(fraction) from the hardware exp2 (V_EXP_F32) on
+// [-0.5, 0.5] and the integer part applied exactly by ldexp.  For the
+// exponents used here (|y| <= 3) the result is within about 3e-7 relative
+// of the exact power; 0^y (y > 0) is 0 and negative, infinite and NaN bases
+// take powf.
+__device__ inline float pow_pos(float x, float y) {
+  if (!(x > 0.0f && x <= 3.40282347e38f)) return x == 0.0f && y > 0.0f ? 0.0f : powf(x, y);
+  const int e = __builtin_amdgcn_frexp_expf(x);
+  const float m = __builtin_amdgcn_frexp_mantf(x);
+  const double t = static_cast<double>(y) * (static_cast<double>(e) + static_cast<double>(__builtin_amdgcn_logf(m)));
+  const double n = rint(t);
+  return ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(t - n)), static_cast<int>(n));
+}
+
 // Fuel::eta_m (firelib.rs:1060-1076)
 __device__ inline float eta_m(const DevFuel &f, bool alive, float life_moist, float wfmd) {
   float life_mext;
@@ -68,25 +86,30 @@ __device__ inline bool burn_cell(const DevFuel *cat, uint32_t code, float d1, fl
   // calculate_wind_dependent_vars / wind_slope_situation (firelib.rs:1078-1167)
   const float ws = ws_ms / kFtMinPerMs;
   const float phi_slope = f.slope_k * slope * slope;
-  const float phi_wind = f.wind_k * powf(ws, f.wind_b);
+  const float phi_wind = f.wind_k * pow_pos(ws, f.wind_b);
   const float phi_ew = phi_slope + phi_wind;
   const float upslope = aspect >= kPi ? aspect - kPi : aspect + kPi;
-  float ew, s, a;
-  bool limit = true;
+  // Each situation sets s and a, and ew either directly or, for the two
+  // that need (phi * wind_e)^(1 / wind_b), through pe (one power call site,
+  // so a wave runs it once whichever of the two its lanes take).
+  float ew = 0.0f, s, a, pe = 0.0f;
+  bool limit = true, ew_pow = false;
   if (speed0 < kSmidgen) {  // NoSpread
-    ew = 0.0f; s = 0.0f; a = 0.0f; limit = false;
+    s = 0.0f; a = 0.0f; limit = false;
   } else if (phi_ew < kSmidgen) {  // NoSlopeNoWind
-    ew = 0.0f; s = speed0; a = 0.0f; limit = false;
+    s = speed0; a = 0.0f; limit = false;
   } else if (slope < kSmidgen) {  // WindNoSlope
     ew = ws; s = speed0 * (1.0f + phi_ew); a = waz;
   } else if (ws_ms < kSmidgen || fabsf(upslope - waz) < kSmidgen) {  // SlopeNoWind | UpSlope
-    ew = powf(phi_ew * f.wind_e, f.wind_b_inv); s = speed0 * (1.0f + phi_ew); a = upslope;
+    pe = phi_ew; ew_pow = true; s = speed0 * (1.0f + phi_ew); a = upslope;
   } else {  // CrossSlope
     const float wind_rate = speed0 * phi_wind;
     const float slope_rate = speed0 * phi_slope;
     const float split = upslope <= waz ? waz - upslope : 2.0f * kPi - upslope + waz;
-    const float x = slope_rate + wind_rate * cosf(split);
-    const float y = wind_rate * sinf(split);
+    float sn, cs;
+    sincosf(split, &sn, &cs);
+    const float x = slope_rate + wind_rate * cs;
+    const float y = wind_rate * sn;
     const float rv = sqrtf(x * x + y * y);
     const float smax2 = speed0 + rv;
     const float phi2 = smax2 / speed0 - 1.0f;
@@ -98,11 +121,12 @@ __device__ inline bool burn_cell(const DevFuel *cat, uint32_t code, float d1, fl
     else split2 = kPi + al;
     float az2 = upslope + split2;
     if (az2 > 2.0f * kPi) az2 = az2 - 2.0f * kPi;
-    ew = powf(phi2 * f.wind_e, f.wind_b_inv); s = smax2; a = az2;
+    pe = phi2; ew_pow = true; s = smax2; a = az2;
   }
+  if (ew_pow) ew = pow_pos(pe * f.wind_e, f.wind_b_inv);
   const float max_wind = 0.9f * rx_int;
   if (limit && ew > max_wind) {  // check_wind_limit
-    s = speed0 * (1.0f + f.wind_k * powf(max_wind, f.wind_b));
+    s = speed0 * (1.0f + f.wind_k * pow_pos(max_wind, f.wind_b));
     ew = max_wind;
   }
   const float lw = 1.0f + 0.002840909f * ew;  // Fuel::eccentricity

@@ -1684,10 +1684,17 @@ __global__ void block_bits(const uint32_t *bits, unsigned long long *blk, uint32
   }
 }
 
-// find_boundaries' predicate for cell (x, y) (propag.cu:345-375,296-302):
-// an in-bounds neighbour has a dissimilar fire that is faster.  The
-// similarity thresholds are compared in double, as the reference's f64
-// constants are.
+// find_boundaries' test of one neighbour (propag.cu:345-375,296-302): its
+// fire is dissimilar (similar_fires compares in double, as the reference's
+// f64 thresholds are, azimuths without wrap-around) and faster.
+__device__ inline bool boundary_pair(float s0, float a0, float e0, float s1, float a1, float e1) {
+  const bool similar = static_cast<double>(fabsf(s1 - s0)) < 0.1 &&
+                       static_cast<double>(fabsf(a1 - a0)) < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
+                       static_cast<double>(fabsf(e1 - e0)) < 0.05;
+  return !similar && s0 < s1;
+}
+
+// The predicate for cell (x, y): some in-bounds neighbour passes the test.
 __device__ inline bool boundary_at(const Geom &g, const float *__restrict__ speed, const float *__restrict__ az,
                                    const float *__restrict__ ecc, int x, int y) {
   const size_t i = static_cast<size_t>(x) + static_cast<size_t>(y) * g.W;
@@ -1699,11 +1706,7 @@ __device__ inline bool boundary_at(const Geom &g, const float *__restrict__ spee
       const int nx = x + di, ny = y + dj;
       if (nx < 0 || ny < 0 || nx >= g.W || ny >= g.H) continue;
       const size_t j = static_cast<size_t>(nx) + static_cast<size_t>(ny) * g.W;
-      const float s1 = speed[j], a1 = az[j], e1 = ecc[j];
-      const bool similar = static_cast<double>(fabsf(s1 - s0)) < 0.1 &&
-                           static_cast<double>(fabsf(a1 - a0)) < (5.0 * (2 * 3.14159265358979323846) / 360.0) &&
-                           static_cast<double>(fabsf(e1 - e0)) < 0.05;
-      changed = changed || (!similar && s0 < s1);
+      changed = changed || boundary_pair(s0, a0, e0, speed[j], az[j], ecc[j]);
     }
   }
   return changed;
@@ -1781,11 +1784,35 @@ __global__ void seed_kernel(Seed a) { seed_ignitions(a.sc, a.si, a.q, a.ign, a.t
 __global__ __launch_bounds__(256) void bits_fused(Geom g, const float *speed, const float *az, const float *ecc,
                                                  uint32_t *bits, unsigned long long *blk, uint32_t bw, int seed,
                                                  Seed sd) {
+  // The 34 x 10 fire window (tile and one-cell ring) in LDS; cells outside
+  // the raster hold a NaN speed, which makes every comparison with them
+  // false, as skipping them does.
+  constexpr int kHW = 34, kHH = 10;
+  __shared__ float hs[kHH][kHW], ha[kHH][kHW], he[kHH][kHW];
   __shared__ uint32_t rows[8];
-  const int c = static_cast<int>(threadIdx.x & 31u), r = static_cast<int>(threadIdx.x >> 5);
   const int x0 = static_cast<int>(blockIdx.x) * 32, y0 = static_cast<int>(blockIdx.y) * 8;
+  for (int k = static_cast<int>(threadIdx.x); k < kHW * kHH; k += 256) {
+    const int hy = k / kHW, hx = k % kHW;
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+    const bool inb = gx >= 0 && gy >= 0 && gx < g.W && gy < g.H;
+    const size_t gi = inb ? static_cast<size_t>(gx) + static_cast<size_t>(gy) * g.W : 0;
+    hs[hy][hx] = inb ? speed[gi] : __builtin_nanf("");
+    ha[hy][hx] = inb ? az[gi] : 0.0f;
+    he[hy][hx] = inb ? ecc[gi] : 0.0f;
+  }
+  __syncthreads();
+  const int c = static_cast<int>(threadIdx.x & 31u), r = static_cast<int>(threadIdx.x >> 5);
   const int y = y0 + r;
-  const bool b = y < g.H && boundary_at(g, speed, az, ecc, x0 + c, y);
+  bool b = false;
+  if (y < g.H) {
+    const float s0 = hs[r + 1][c + 1], a0 = ha[r + 1][c + 1], e0 = he[r + 1][c + 1];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      if (q == 4) continue;
+      const int hx = c + q % 3, hy = r + q / 3;
+      b = b || boundary_pair(s0, a0, e0, hs[hy][hx], ha[hy][hx], he[hy][hx]);
+    }
+  }
   const unsigned long long m = __ballot(b);
   const unsigned lane = threadIdx.x & 63u;
   if (lane == 0u || lane == 32u) {
@@ -1890,16 +1917,13 @@ __global__ __launch_bounds__(256) void admit_cells(AdmitArgs a) {
 }
 
 // post_propagate (propag.cu:110-117) plus unpacking into the caller's rasters.
-__global__ void post(Scen sc, int final_buf, const ScenOut *out, const float *speed, float *time, uint16_t *rx,
-                     uint16_t *ry) {
+// post_propagate's cell rule plus unpacking into the caller's rasters, for
+// cells [i0, n) with stride `step`.
+__device__ inline void post_cells(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *rx,
+                                  uint16_t *ry, size_t i0, size_t step) {
   const size_t n = static_cast<size_t>(sc.g.W) * sc.g.H;
-  if (out) {
-    const ScenOut o = *out;
-    final_buf = scen_final_buf(sc, o);
-  }
   const uint2 *st = final_buf ? sc.st[1] : sc.st[0];
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+  for (size_t i = i0; i < n; i += step) {
     const uint2 w = st[i];
     float t = __uint_as_float(w.x);
     // A cell never reached keeps the caller's references untouched.
@@ -1910,6 +1934,28 @@ __global__ void post(Scen sc, int final_buf, const ScenOut *out, const float *sp
     if (speed[i] == 0.0f || t > sc.g.max_time) t = FLT_MAX;
     time[i] = t;
   }
+}
+
+__global__ void post(Scen sc, int final_buf, const ScenOut *out, const float *speed, float *time, uint16_t *rx,
+                     uint16_t *ry) {
+  if (out) {
+    const ScenOut o = *out;
+    final_buf = scen_final_buf(sc, o);
+  }
+  post_cells(sc, final_buf, speed, time, rx, ry, static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x,
+             static_cast<size_t>(gridDim.x) * blockDim.x);
+}
+
+// The post-pass of several finished scenarios of a batching context in one
+// launch: workgroup row y serves items[y], whose descriptor is scen[slot].
+struct PostBatch {
+  PostItem item[kPostBatch];
+};
+__global__ __launch_bounds__(256) void post_batch(const Scen *scen, PostBatch b) {
+  const PostItem &it = b.item[blockIdx.y];
+  const Scen sc = scen[it.slot];
+  post_cells(sc, it.final_buf, it.speed, it.time, it.rx, it.ry,
+             static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x, static_cast<size_t>(gridDim.x) * blockDim.x);
 }
 
 // Initial state of a scenario from point ignitions (propag.rs:433-449 for
@@ -2305,6 +2351,23 @@ pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream) {
   if (b > 2048) b = 2048;
   hipLaunchKernelGGL(fill_u32, dim3(static_cast<unsigned>(b ? b : 1)), dim3(256), 0, stream, p, n, v);
   PG_HIP(hipGetLastError());
+  return PG_OK;
+}
+
+pg_status scen_finish_batch(const DeviceInfo *dev, const Scen *d_scen, const PostItem *items, size_t n_items,
+                            size_t max_cells, hipStream_t stream) {
+  for (size_t i0 = 0; i0 < n_items; i0 += kPostBatch) {
+    const size_t m = std::min<size_t>(kPostBatch, n_items - i0);
+    PostBatch b{};
+    for (size_t j = 0; j < m; ++j) b.item[j] = items[i0 + j];
+    // enough workgroups per scenario to fill the device with all m of them
+    size_t bx = (max_cells + 255) / 256;
+    const size_t cap = std::max<size_t>(1, static_cast<size_t>(dev->cu_count) * 16 / m);
+    if (bx > cap) bx = cap;
+    hipLaunchKernelGGL(post_batch, dim3(static_cast<unsigned>(bx), static_cast<unsigned>(m)), dim3(256), 0, stream,
+                       d_scen, b);
+    PG_HIP(hipGetLastError());
+  }
   return PG_OK;
 }
 

@@ -158,6 +158,18 @@ pg_status scen_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream);
 pg_status scen_finish(const Scen &sc, int final_buf, const float *speed, float *time, uint16_t *refs_x,
                       uint16_t *refs_y, hipStream_t stream, const ScenOut *out = nullptr);
 
+// The post-pass of finished scenarios of a batching context, whose
+// descriptors are on the device (d_scen[slot]), kPostBatch per launch.
+struct PostItem {
+  int slot, final_buf;
+  const float *speed;
+  float *time;
+  uint16_t *rx, *ry;
+};
+constexpr int kPostBatch = 32;
+pg_status scen_finish_batch(const DeviceInfo *dev, const Scen *d_scen, const PostItem *items, size_t n_items,
+                            size_t max_cells, hipStream_t stream);
+
 // Buffer that holds a finished scenario's final state.
 __host__ __device__ inline int scen_final_buf(const Scen &sc, const ScenOut &o) {
   if (o.capped) return static_cast<int>((static_cast<unsigned>(sc.k0) + sc.max_iters) & 1u);
