@@ -555,7 +555,10 @@ struct WalkQueue {
   unsigned n;  // queued walks (wave-uniform)
   unsigned long long *e;  // this wave's kEntryQ list entries for the next sweep (LDS)
   unsigned ne;            // held entries (wave-uniform)
+  uint32_t changed_si = 0xFFFFFFFFu;  // the scenario whose last_change this wave set in this sweep
+  unsigned long long *lc = nullptr;  // the workgroup's (scenario, sweep) pairs stored (LDS, kLastChange), or null
 };
+constexpr int kLastChange = 16;  // direct-mapped by scenario
 
 // Runs the queued walks, one per lane, and writes those cells' states.
 __device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
@@ -1183,9 +1186,18 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     const uint32_t marks = next_rest<kDefer, TW>(a.q, sc, si, k, nt, wq, lane);
     if (lane == 0) cnt.marks += marks;
     // Register this tile as a dependent of each new far reference's tile
-    // and mark the reference cells, one set of atomics per distinct
-    // reference tile.
+    // and mark the reference cells: each distinct reference tile's group is
+    // gathered in its first lane, which then reads that tile's box and
+    // far-reference bits and issues only the atomics that change them.
+    // Boxes only grow and bits are only set during a launch, so any value
+    // read (also a stale one) lies within the current one: skipping an
+    // update it already covers is exact, and the many tiles whose cells take
+    // references in the same few tiles (ignitions) no longer queue atomics
+    // on the same words.
     unsigned long long m = __ballot(far_dep);
+    bool lead = false;
+    uint32_t g_rt = 0, g_lo = 0, g_hi = 0;
+    uint4 g_box = make_uint4(0u, 0u, 0u, 0u);
     while (m) {
       const int l = __builtin_ctzll(m);
       const uint32_t rt0 = __shfl(rtile, l);
@@ -1198,20 +1210,35 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         hi |= __shfl_xor(hi, off);
       }
       if (lane == l) {
-        uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[rt0]);
-        if (lo) or_relaxed(r, lo);
-        if (hi) or_relaxed(r + 1, hi);
+        lead = true;
+        g_rt = rt0;
+        g_lo = lo;
+        g_hi = hi;
         // the group's cells' bounding box, from the ballot (lane = x + TW y)
         unsigned long long cols = gm;
 #pragma unroll
         for (int sh = 32; sh >= TW; sh >>= 1) cols |= cols >> sh;
         cols &= TW == 64 ? ~0ull : (1ull << TW) - 1ull;
-        uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[rt0]);
-        min_relaxed(b + 0, static_cast<uint32_t>(ox + __builtin_ctzll(cols)));
-        min_relaxed(b + 1, static_cast<uint32_t>(oy + __builtin_ctzll(gm) / TW));
-        max_relaxed(b + 2, static_cast<uint32_t>(ox + 63 - __builtin_clzll(cols)));
-        max_relaxed(b + 3, static_cast<uint32_t>(oy + (63 - __builtin_clzll(gm)) / TW));
+        g_box = make_uint4(static_cast<uint32_t>(ox + __builtin_ctzll(cols)),
+                           static_cast<uint32_t>(oy + __builtin_ctzll(gm) / TW),
+                           static_cast<uint32_t>(ox + 63 - __builtin_clzll(cols)),
+                           static_cast<uint32_t>(oy + (63 - __builtin_clzll(gm)) / TW));
       }
+    }
+    if (lead) {
+      uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[g_rt]);
+      uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[g_rt]);
+      uint32_t cur[6];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cur[q] = a.cached_stage ? ld_u32_c(b + q) : ld_u32(b + q);
+      cur[4] = a.cached_stage ? ld_u32_c(r) : ld_u32(r);
+      cur[5] = a.cached_stage ? ld_u32_c(r + 1) : ld_u32(r + 1);
+      if (g_lo & ~cur[4]) or_relaxed(r, g_lo);
+      if (g_hi & ~cur[5]) or_relaxed(r + 1, g_hi);
+      if (g_box.x < cur[0]) min_relaxed(b + 0, g_box.x);
+      if (g_box.y < cur[1]) min_relaxed(b + 1, g_box.y);
+      if (g_box.z > cur[2]) max_relaxed(b + 2, g_box.z);
+      if (g_box.w > cur[3]) max_relaxed(b + 3, g_box.w);
     }
   }
   if (lane == 0) ++cnt.tiles;
@@ -1255,7 +1282,22 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
   }
   if (tile >= static_cast<uint32_t>(sc.ntx) * static_cast<uint32_t>(sc.nty)) return false;
   const bool tc = wave_tile<kCache, kLdsBits, TW>(a, sc, si, lds_off, static_cast<int>(tile), k, s, wq, cnt);
-  if (tc && lane == 0) st_i32(&a.out[si].last_change, k);
+  // last_change is stored once per wave, scenario and sweep, and in the
+  // persistent kernel once per workgroup as far as its direct-mapped table
+  // of (scenario, sweep) pairs tells: in a launch of one large scenario
+  // every changed tile would store to the same word, and such stores queue
+  // up behind each other at that word's memory channel.
+  if (tc && wq.changed_si != si) {
+    if (lane == 0) {
+      const unsigned long long key = (static_cast<unsigned long long>(si) << 32) | static_cast<uint32_t>(k);
+      unsigned long long *slot = wq.lc ? &wq.lc[si % kLastChange] : nullptr;
+      if (!slot || *slot != key) {
+        if (slot) *slot = key;
+        st_i32(&a.out[si].last_change, k);
+      }
+    }
+    wq.changed_si = si;
+  }
   return tc;
 }
 
@@ -1267,8 +1309,9 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
 // first empty slot it draws.
 template <bool kCache, bool kLdsBits, int TW>
 __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds<TW> &s, DeferLds *dq,
-                               Counters &cnt) {
+                               Counters &cnt, unsigned long long *last_change) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
+  wq.lc = last_change;
   const unsigned lane = threadIdx.x & (kWave - 1);
   // Workgroups fall in nx classes, w in class w % nx; class c takes the
   // sub-lists c, c + nx, ... and its workgroups share them, workgroup w
@@ -1517,6 +1560,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   __shared__ unsigned s_changed[2];
   __shared__ unsigned s_next[2][kXcds];  // entries drawn from each sub-list in sweep k (by parity, like s_changed)
   __shared__ unsigned long long s_ph[2][4];
+  __shared__ unsigned long long s_lastc[kLastChange];  // (scenario << 32 | sweep) of the last last_change store
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
   // Dynamic LDS: (LDS bitset) the boundary bitset at offset 0, so a walk's
@@ -1530,6 +1574,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
   if (threadIdx.x < 2 * kXcds) s_next[threadIdx.x / kXcds][threadIdx.x % kXcds] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
+  if (threadIdx.x < kLastChange) s_lastc[threadIdx.x] = ~0ull;
   if (kLdsBits) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
   }
@@ -1537,7 +1582,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   Counters cnt{};
   uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
   for (int k = a.k_begin;; ++k) {
-    const bool changed = sweep_wave_dyn<kCache, kLdsBits, TW>(a, s_next[k & 1], lds_off, k, s, dq, cnt);
+    const bool changed = sweep_wave_dyn<kCache, kLdsBits, TW>(a, s_next[k & 1], lds_off, k, s, dq, cnt, s_lastc);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
     if (a.prof && lane == 0) {
 #pragma unroll
@@ -2536,7 +2581,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     // Waves per workgroup: as many tile slices as fit beside the bitset when
     // the bitset can live in LDS (one workgroup per CU), else 16 waves.
     const size_t bits_bytes = ((L.lds_bits_words + 3) & ~size_t{3}) * sizeof(uint32_t);  // padded as the kernel lays it out
-    const size_t static_lds = 256;
+    const size_t static_lds = 320;  // the persistent kernel's own (static) LDS, rounded up
     int waves = 16;
     a.lds_bits = 0;
     const size_t slice = tile_lds_bytes(tw);
