@@ -1228,11 +1228,15 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     if (lead) {
       uint32_t *r = reinterpret_cast<uint32_t *>(&sc.isref[g_rt]);
       uint32_t *b = reinterpret_cast<uint32_t *>(&sc.box[g_rt]);
-      uint32_t cur[6];
+      // (The one-scenario LDS-bitset kernel runs its sweeps as one chain of
+      // round trips and takes the atomics without the read.)
+      uint32_t cur[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, 0u};
+      if (!kLdsBits) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) cur[q] = a.cached_stage ? ld_u32_c(b + q) : ld_u32(b + q);
-      cur[4] = a.cached_stage ? ld_u32_c(r) : ld_u32(r);
-      cur[5] = a.cached_stage ? ld_u32_c(r + 1) : ld_u32(r + 1);
+        for (int q = 0; q < 4; ++q) cur[q] = a.cached_stage ? ld_u32_c(b + q) : ld_u32(b + q);
+        cur[4] = a.cached_stage ? ld_u32_c(r) : ld_u32(r);
+        cur[5] = a.cached_stage ? ld_u32_c(r + 1) : ld_u32(r + 1);
+      }
       if (g_lo & ~cur[4]) or_relaxed(r, g_lo);
       if (g_hi & ~cur[5]) or_relaxed(r + 1, g_hi);
       if (g_box.x < cur[0]) min_relaxed(b + 0, g_box.x);
