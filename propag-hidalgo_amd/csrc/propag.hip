@@ -556,9 +556,13 @@ struct WalkQueue {
   unsigned long long *e;  // this wave's kEntryQ list entries for the next sweep (LDS)
   unsigned ne;            // held entries (wave-uniform)
   uint32_t changed_si = 0xFFFFFFFFu;  // the scenario whose last_change this wave set in this sweep
-  unsigned long long *lc = nullptr;  // the workgroup's (scenario, sweep) pairs stored (LDS, kLastChange), or null
+  unsigned long long *lc = nullptr;  // the workgroup's (scenario, sweep) pairs stored (LDS), or null
+  uint32_t lc_mask = 0;              // its entries - 1 (direct-mapped by scenario)
 };
-constexpr int kLastChange = 16;  // direct-mapped by scenario
+// Entries of that table: the one-scenario LDS-bitset kernel has little LDS
+// to spare; a batched launch keeps one per scenario slot it is likely to meet.
+template <bool kLdsBits>
+constexpr int last_change_entries() { return kLdsBits ? 16 : 256; }
 
 // Runs the queued walks, one per lane, and writes those cells' states.
 __device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
@@ -1294,7 +1298,7 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
   if (tc && wq.changed_si != si) {
     if (lane == 0) {
       const unsigned long long key = (static_cast<unsigned long long>(si) << 32) | static_cast<uint32_t>(k);
-      unsigned long long *slot = wq.lc ? &wq.lc[si % kLastChange] : nullptr;
+      unsigned long long *slot = wq.lc ? &wq.lc[si & wq.lc_mask] : nullptr;
       if (!slot || *slot != key) {
         if (slot) *slot = key;
         st_i32(&a.out[si].last_change, k);
@@ -1316,6 +1320,7 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
                                Counters &cnt, unsigned long long *last_change) {
   WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
   wq.lc = last_change;
+  wq.lc_mask = static_cast<uint32_t>(last_change_entries<kLdsBits>() - 1);
   const unsigned lane = threadIdx.x & (kWave - 1);
   // Workgroups fall in nx classes, w in class w % nx; class c takes the
   // sub-lists c, c + nx, ... and its workgroups share them, workgroup w
@@ -1564,6 +1569,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   __shared__ unsigned s_changed[2];
   __shared__ unsigned s_next[2][kXcds];  // entries drawn from each sub-list in sweep k (by parity, like s_changed)
   __shared__ unsigned long long s_ph[2][4];
+  constexpr int kLastChange = last_change_entries<kLdsBits>();
   __shared__ unsigned long long s_lastc[kLastChange];  // (scenario << 32 | sweep) of the last last_change store
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
@@ -1578,7 +1584,7 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   if (threadIdx.x < 2) s_changed[threadIdx.x] = 0;
   if (threadIdx.x < 2 * kXcds) s_next[threadIdx.x / kXcds][threadIdx.x % kXcds] = 0;
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
-  if (threadIdx.x < kLastChange) s_lastc[threadIdx.x] = ~0ull;
+  for (unsigned i = threadIdx.x; i < static_cast<unsigned>(kLastChange); i += blockDim.x) s_lastc[i] = ~0ull;
   if (kLdsBits) {
     for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
   }
