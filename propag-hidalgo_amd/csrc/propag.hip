@@ -266,7 +266,8 @@ __device__ inline uint32_t make_ref(int x, int y) {
 }
 
 // Dynamic LDS of the persistent kernel: the boundary bitset when it fits
-// (at offset 0), then the per-wave tile slices.
+// (first, right after the kernel's few static words), then the per-wave
+// tile slices.
 extern __shared__ __attribute__((aligned(16))) uint32_t dyn_lds[];
 
 // Where a walk reads the boundary bits.  From global memory (L2) it reads a
@@ -1573,8 +1574,8 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   __shared__ unsigned long long s_lastc[kLastChange];  // (scenario << 32 | sweep) of the last last_change store
   const int nw = blockDim.x / kWave, wave = threadIdx.x / kWave;
   const unsigned lane = threadIdx.x & (kWave - 1);
-  // Dynamic LDS: (LDS bitset) the boundary bitset at offset 0, so a walk's
-  // word address needs no base; then nw tile slices and (deferred walks) nw
+  // Dynamic LDS: (LDS bitset) the boundary bitset first; then nw tile
+  // slices and (deferred walks) nw
   // blocks of deferred work (walk queue and held list entries).
   const unsigned bits_words = kLdsBits ? (a.lds_words + 3u) & ~3u : 0u;  // 16-byte aligned slices
   TileLds<TW> *slices = reinterpret_cast<TileLds<TW> *>(dyn_lds + bits_words);
