@@ -114,6 +114,30 @@ def test_batch_matches_alone_and_oracle(oracle_lib):
     print(f"batched 6 scenarios: {stats['iterations']} sweeps, {stats['tiles_visited']} tile visits")
 
 
+@pytest.mark.parametrize("w,h", [(160, 96), (150, 97)])
+def test_admission_paths_match_alone(w, h):
+    """Constant-weather scenarios with at most 16 ignitions are admitted in
+    two launches (burn, coefficients and state in one pass; the bitset pass
+    seeds the ignitions), others through the general set-up (burn pass,
+    device ignition list, prep): both give the one-per-call results, on a
+    width that is a multiple of 32 (fused bitset pass) and one that is not."""
+    import py_propag as pp
+
+    land = _terrain(23, w, h)
+    many = _ignitions(5, w, h, 20)  # above the 16 passed by value
+    few = _ignitions(6, w, h, 16)
+    cases = [{"wx": _weather(40), "ign": few, "max_time": 5400.0},
+             {"wx": _weather(41), "ign": many, "max_time": 5400.0},
+             {"wx": _weather(42, w, h, field=True), "ign": few[:3], "max_time": 3600.0}]
+    with pp.Context() as ctx:
+        tid = ctx.add_terrain(_geo(pp, w, h), land["fuel"], land["slope"], land["aspect"])
+        ids = [ctx.admit(tid, c["wx"], c["ign"], c["max_time"]) for c in cases]
+        fin, _ = ctx.run()
+        assert fin == len(cases)
+        for sid, c in zip(ids, cases):
+            _same(ctx.result(sid), _alone(pp, land, w, h, c["wx"], c["ign"], c["max_time"]))
+
+
 def test_continuous_batching_admits_between_short_launches():
     import py_propag as pp
 

@@ -6,8 +6,10 @@ an angular difference where the fire spreads faster than 1 ft/min (the
 reference's own azimuth waiver, crates/firelib/src/firelib.rs:473-490):
 1e-5 relative to a full turn, widened where the reference's asin(|y|/rv) is
 ill-conditioned (see _az_tolerance; the f32 oracle is itself up to 4e-4 rad
-from FP64 there).  Differences come only from the device libm (powf, sinf, cosf,
-asinf) vs the host's.
+from FP64 there).  Differences come only from the device arithmetic of the
+transcendental functions vs the host's: the device library's sincosf and
+asinf, and for the positive-base powers the kernel's own exp2(y * log2 x)
+(hardware log2 / exp2 with the product in double, about 3e-7 relative).
 """
 import numpy as np
 import pytest
