@@ -85,6 +85,7 @@ struct pg_ctx {
   // since the last upload, which pg_batch_run issues before its launch.
   pg::Scen *h_scen = nullptr;
   int desc_lo = 1 << 30, desc_hi = -1;
+  pg::ScenOut *h_out = nullptr;  // pinned: the launch outcomes are copied here
   pg::TileQueue q{};
   void *q_block = nullptr;
   // Ignition lists of admissions since the last synchronisation (the
@@ -283,6 +284,7 @@ pg_status pg_ctx_create(size_t hbm_budget, pg_ctx **out) {
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_scen), kMaxSlots * sizeof(Scen));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&c->d_out), kMaxSlots * sizeof(ScenOut));
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&c->h_scen), kMaxSlots * sizeof(Scen));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&c->h_out), kMaxSlots * sizeof(ScenOut));
   if (e != hipSuccess) {
     st = hip_error(e, "pg_ctx_create");
     pg_ctx_destroy(c);
@@ -308,6 +310,7 @@ pg_status pg_ctx_destroy(pg_ctx *c) {
   if (c->d_scen) (void)hipFree(c->d_scen);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->h_scen) (void)hipHostFree(c->h_scen);
+  if (c->h_out) (void)hipHostFree(c->h_out);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PG_OK;
@@ -549,8 +552,8 @@ pg_status pg_batch_run(pg_ctx *c, uint32_t max_sweeps, const pg_tuning *tuning, 
   LaunchResult res{};
   pg_status st = run_sweeps(c->dev, L, tuning, &local, &res, c->stream);
   if (st != PG_OK) return fail_running(c, st);
-  std::vector<ScenOut> outs(nslots);
-  PG_HIP(hipMemcpyAsync(outs.data(), c->d_out, nslots * sizeof(ScenOut), hipMemcpyDeviceToHost, c->stream));
+  const ScenOut *outs = c->h_out;
+  PG_HIP(hipMemcpyAsync(c->h_out, c->d_out, nslots * sizeof(ScenOut), hipMemcpyDeviceToHost, c->stream));
   PG_HIP(hipStreamSynchronize(c->stream));
   uint32_t fin = 0;
   std::vector<PostItem> posts;

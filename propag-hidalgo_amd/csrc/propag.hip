@@ -2081,11 +2081,41 @@ struct LaunchGraph {
 // Per device: the buffers of the one-scenario entry point and of a launch
 // (control block, barrier slots, profile rows), each behind its own lock,
 // and the launch graphs (under the launch lock).
+// Pinned host memory for a call's small device-to-host results: copies into
+// pageable memory go through the runtime's staging (the stream is drained
+// first), pinned ones are ordinary asynchronous copies.  The launch part is
+// used under launch_mu, the call part under call_mu.
+struct HostReturn {
+  alignas(128) Control ctl;
+  alignas(128) ScenOut out;
+  alignas(128) Scen desc;  // the call's descriptor, uploaded from here
+};
+
+// A persistent kernel's resident workgroups per CU for a launch shape.
+struct Occupancy {
+  const void *kern;
+  int block;
+  size_t shmem;
+  int per_cu;
+};
+
 struct PerDevice {
   std::mutex call_mu, launch_mu;
   DeviceBuffer call, launch;
   std::vector<LaunchGraph *> graphs;
+  HostReturn *ret = nullptr;  // pinned, or null (then pageable copies)
+  // Timing events kept across calls: [0, 1] the call's (under call_mu),
+  // [2, 3] the relaxation launch's (under launch_mu).
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<Occupancy> occ;  // under launch_mu
 };
+
+// The device's timing event i, created on first use.
+pg_status device_event(PerDevice *pd, int i, hipEvent_t *out) {
+  if (!pd->ev[i]) PG_HIP(hipEventCreate(&pd->ev[i]));
+  *out = pd->ev[i];
+  return PG_OK;
+}
 
 std::mutex g_dev_mu;
 std::vector<PerDevice *> g_dev;
@@ -2093,7 +2123,14 @@ std::vector<PerDevice *> g_dev;
 PerDevice *per_device(int device) {
   std::lock_guard<std::mutex> lk(g_dev_mu);
   if (g_dev.size() <= static_cast<size_t>(device)) g_dev.resize(device + 1, nullptr);
-  if (!g_dev[device]) g_dev[device] = new PerDevice();
+  if (!g_dev[device]) {
+    g_dev[device] = new PerDevice();
+    void *p = nullptr;
+    if (hipHostMalloc(&p, sizeof(HostReturn)) == hipSuccess)
+      g_dev[device]->ret = static_cast<HostReturn *>(p);
+    else
+      (void)hipGetLastError();
+  }
   return g_dev[device];
 }
 
@@ -2170,6 +2207,13 @@ void release_launch_resources() {
       b->base = nullptr;
       b->bytes = 0;
     }
+    if (pd->ret) (void)hipHostFree(pd->ret);
+    pd->ret = nullptr;
+    for (hipEvent_t &e : pd->ev) {
+      if (e) (void)hipEventDestroy(e);
+      e = nullptr;
+    }
+    pd->occ.clear();
   }
   if (have) (void)hipSetDevice(current);
   (void)hipGetLastError();
@@ -2540,9 +2584,18 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
   PG_HIP(hipGetLastError());
 
   hipEvent_t evr0 = nullptr, evr1 = nullptr;
+  hipEvent_t own0 = nullptr, own1 = nullptr;  // created for this launch only (caller's workspace)
   if (!L.untimed) {
-    PG_HIP(hipEventCreate(&evr0));
-    PG_HIP(hipEventCreate(&evr1));
+    if (pd) {
+      st = device_event(pd, 2, &evr0);
+      if (st == PG_OK) st = device_event(pd, 3, &evr1);
+      if (st != PG_OK) return st;
+    } else {
+      PG_HIP(hipEventCreate(&own0));
+      PG_HIP(hipEventCreate(&own1));
+      evr0 = own0;
+      evr1 = own1;
+    }
   }
   struct EventGuard {
     hipEvent_t *e[2];
@@ -2550,7 +2603,7 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       for (hipEvent_t *p : e)
         if (*p) (void)hipEventDestroy(*p);
     }
-  } guard{{&evr0, &evr1}};
+  } guard{{&own0, &own1}};
 
   unsigned book[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned launched_groups = 0;
@@ -2581,8 +2634,10 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
       const pg_status s2 = L.after(L.after_ctx, stream);
       if (s2 != PG_OK) return s2;
     }
-    if (copy_ctl) PG_HIP(hipMemcpyAsync(&hc, a.ctl, sizeof(Control), hipMemcpyDeviceToHost, stream));
+    Control *dst = (pd && pd->ret) ? &pd->ret->ctl : &hc;
+    if (copy_ctl) PG_HIP(hipMemcpyAsync(dst, a.ctl, sizeof(Control), hipMemcpyDeviceToHost, stream));
     PG_HIP(hipStreamSynchronize(stream));
+    if (copy_ctl && dst != &hc) std::memcpy(&hc, dst, sizeof(Control));
     return PG_OK;
   };
   if (tn.mode == 1) {
@@ -2618,7 +2673,16 @@ pg_status run_sweeps(const DeviceInfo *dev, const Launch &L, const pg_tuning *tu
     const int block = waves * kWave;
     void (*kern)(Args) = persistent_kernel(tw, L.cache != 0, a.lds_bits != 0);
     int per_cu = 0;
-    PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, shmem));
+    const Occupancy *known = nullptr;
+    if (pd)
+      for (const Occupancy &o : pd->occ)
+        if (o.kern == reinterpret_cast<const void *>(kern) && o.block == block && o.shmem == shmem) known = &o;
+    if (known) {
+      per_cu = known->per_cu;
+    } else {
+      PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, shmem));
+      if (pd) pd->occ.push_back(Occupancy{reinterpret_cast<const void *>(kern), block, shmem, per_cu});
+    }
     if (per_cu < 1) return set_error(PG_ERR_INVALID, "persistent kernel does not fit on a CU");
     unsigned groups = cu * static_cast<unsigned>(per_cu);
     if (groups > max_groups) groups = static_cast<unsigned>(max_groups);
@@ -2784,6 +2848,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   const size_t need = call_ws_bytes(sc, nsub);
   char *base = nullptr;
   std::unique_lock<std::mutex> lk;
+  HostReturn *hr = nullptr;  // pinned descriptor / outcome staging (library workspace only)
   if (space) {
     if (need > space->call_bytes) return set_error(PG_ERR_INVALID, "raster too large for the workspace");
     base = space->call;
@@ -2793,6 +2858,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
     st = ensure(&pd->call, need);
     if (st != PG_OK) return st;
     base = static_cast<char *>(pd->call.base);
+    hr = pd->ret;
   }
   const ScenBuffers b = scen_carve(base + o_scen, W, H);
   sc.coef = b.coef;
@@ -2817,20 +2883,16 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
   const bool timed = space == nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (timed) {
-    PG_HIP(hipEventCreate(&ev0));
-    PG_HIP(hipEventCreate(&ev1));
+    PerDevice *pd = per_device(dev->device);  // call_mu is held
+    st = device_event(pd, 0, &ev0);
+    if (st == PG_OK) st = device_event(pd, 1, &ev1);
+    if (st != PG_OK) return st;
   }
-  struct EventGuard {
-    hipEvent_t *e[2];
-    ~EventGuard() {
-      for (hipEvent_t *p : e)
-        if (*p) (void)hipEventDestroy(*p);
-    }
-  } guard{{&ev0, &ev1}};
   if (ev0) PG_HIP(hipEventRecord(ev0, stream));
   st = scen_prepare(dev, sc, 0, d_out, q, speed, az, ecc, time, refs_x, refs_y, boundaries, false, stream, true);
   if (st != PG_OK) return st;
-  PG_HIP(hipMemcpyAsync(d_scen, &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
+  if (hr) hr->desc = sc;  // free: the previous call on this device has synchronised
+  PG_HIP(hipMemcpyAsync(d_scen, hr ? &hr->desc : &sc, sizeof(Scen), hipMemcpyHostToDevice, stream));
   // Post-pass and outcome right behind the relaxation, so the call waits on
   // its stream once: the post kernel reads the outcome on the device.
   struct After {
@@ -2842,7 +2904,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
     hipEvent_t ev1;
   };
   ScenOut o{};
-  After aft{&sc, d_out, &o, speed, time, refs_x, refs_y, ev1};
+  After aft{&sc, d_out, hr ? &hr->out : &o, speed, time, refs_x, refs_y, ev1};
 
   Launch L{};
   L.scen = d_scen;
@@ -2874,6 +2936,7 @@ static pg_status propagate_call(const DeviceInfo *dev, const Settings *settings,
     (void)hipStreamSynchronize(stream);
     return st;
   }
+  if (hr) o = hr->out;  // run_sweeps has synchronised the stream
   if (o.err & 2u) return set_error(PG_ERR_INVALID, "a burned cell (time < FLT_MAX) has a reference outside the raster");
   float ms = 0.0f;
   if (ev0) PG_HIP(hipEventElapsedTime(&ms, ev0, ev1));
