@@ -282,6 +282,7 @@ struct BitView {
   const unsigned long long *blk;  // blocked copy (global form)
   unsigned lds_off;               // unused: the LDS form's row-major bitset starts dyn_lds
   int W, bw, bh;
+  int ncell;                      // W x H (the LDS form's clamp)
 };
 
 template <bool kLdsBits>
@@ -298,6 +299,16 @@ __device__ inline bool bit_at(const BitView &bv, int x, int y) {
   const unsigned long long w = G(bv.blk)[static_cast<uint32_t>((x >> 3) + (y >> 3) * bv.bw)];
   return (w >> ((x & 7) | ((y & 7) << 3))) & 1ull;
 }
+
+// The walks address the LDS bitset by absolute LDS byte address: a cell
+// index plus lds_bit_base() (the bitset's LDS address in bits, a multiple of
+// 32) gives the word's byte address as (id >> 3) & ~3 with the cell's bit
+// at id & 31, so no base is added per cell.
+typedef const __attribute__((address_space(3))) uint32_t lds_cu32;
+__device__ inline uint32_t lds_bit_base() {
+  return static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) uint32_t *)dyn_lds)) << 3;
+}
+__device__ inline uint32_t lds_bit_word(uint32_t id) { return *(lds_cu32 *)(size_t)((id >> 3) & ~3u); }
 
 // (double)d < 1e-9 for a float d >= 0 is exactly d <= this float (the float
 // nearest 1e-9 lies below it and the next one above).
@@ -329,68 +340,80 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const BitView &bv, 
   // cannot end the walk; pu / pv lift it out of the all-live test below.
   const int pu = ddx == 0 ? (1 << 24) : 0, pv = ddy == 0 ? (1 << 24) : 0;
   if constexpr (kLdsBits) {
-    const int iy = sy * bv.W;
-    int idx = fx + fy * bv.W;
+    // The cell visited with ru, rv steps left has the index
+    // tc - sx*ru - sy*W*rv (24-bit products: |ru|, |rv|, W < 2^15 and the
+    // raster below 2^30 cells), here biased by the bitset's LDS address.  Taking it from the counts keeps them
+    // current at every step (one subtract-with-borrow each) instead of
+    // carrying a running index beside them.
+    const uint32_t base = lds_bit_base();
+    const int tc = tx + ty * bv.W + static_cast<int>(base);
+    const int nsx = -sx, nsw = -sy * bv.W;
+    const uint32_t last = static_cast<uint32_t>(bv.ncell - 1) + base;
     for (;;) {
+      uint32_t id[8];
       // ru and rv drop by at most one per cell, so with both at 7 or more all
       // eight cells of the batch lie on the walk: no per-cell end test, and
       // the bits are only OR-ed (bit 0 of each shifted word) until one is
       // set.  Taken only when every walking lane of the wave qualifies, so
       // the wave never runs both forms of the batch.
       if (__ballot(min(ru + pu, rv + pv) >= 7) == __activemask()) {
-        uint32_t id[8];
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
-          id[b] = static_cast<uint32_t>(idx);
+          id[b] = static_cast<uint32_t>(tc + __mul24(nsx, ru) + __mul24(nsw, rv));
           const float d = tmx - tmy;
           const bool xs = d <= kTie, ys = !(d < -kTie);
           ru -= xs ? 1 : 0;
           rv -= ys ? 1 : 0;
-          idx += (xs ? sx : 0) + (ys ? iy : 0);
           tmx = xs ? tmx + dlx : tmx;
           tmy = ys ? tmy + dly : tmy;
         }
         uint32_t acc = 0;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) acc |= dyn_lds[id[b] >> 5] >> (id[b] & 31u);
+        for (int b = 0; b < 8; ++b) acc |= lds_bit_word(id[b]) >> (id[b] & 31u);
         if (acc & 1u) {
           uint32_t hits = 0;
 #pragma unroll
-          for (int b = 0; b < 8; ++b) hits |= ((dyn_lds[id[b] >> 5] >> (id[b] & 31u)) & 1u) << b;
+          for (int b = 0; b < 8; ++b) hits |= ((lds_bit_word(id[b]) >> (id[b] & 31u)) & 1u) << b;
           steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
           return true;
         }
         steps += 8;
         continue;
       }
-      uint32_t id[8];
-      int live[8];
+      // The walk may end within the batch.  A cell past its end (at most
+      // seven) reads an in-raster index (the unsigned clamp) and is dropped
+      // by the dead mask: bit 7 - b is set when cell b's counts are past the
+      // end, and since the counts never grow the dead cells are a suffix.
+      uint32_t dead = 0;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
-        live[b] = ru | rv;
-        id[b] = live[b] >= 0 ? static_cast<uint32_t>(idx) : 0u;
+        id[b] = min(static_cast<uint32_t>(tc + __mul24(nsx, ru) + __mul24(nsw, rv)), last);
+        dead = __builtin_amdgcn_alignbit(dead, static_cast<uint32_t>(ru | rv), 31);
         const float d = tmx - tmy;
         const bool xs = d <= kTie, ys = !(d < -kTie);
         ru -= xs ? 1 : 0;
         rv -= ys ? 1 : 0;
-        idx += (xs ? sx : 0) + (ys ? iy : 0);
         tmx = xs ? tmx + dlx : tmx;
         tmy = ys ? tmy + dly : tmy;
       }
+      uint32_t acc = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc |= lds_bit_word(id[b]) >> (id[b] & 31u);
+      if (!(acc & 1u) && (ru | rv) >= 0) {
+        steps += 8;
+        continue;
+      }
+      const uint32_t live = 8u - static_cast<uint32_t>(__builtin_popcount(dead & 0xffu));
       uint32_t hits = 0;
 #pragma unroll
-      for (int b = 0; b < 8; ++b)
-        hits |= (((dyn_lds[id[b] >> 5] >> (id[b] & 31u)) & 1u) && live[b] >= 0) ? (1u << b) : 0u;
+      for (int b = 0; b < 8; ++b) hits |= ((lds_bit_word(id[b]) >> (id[b] & 31u)) & 1u) << b;
+      hits &= (1u << live) - 1u;
       if (hits) {
         steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
         return true;
       }
-      if ((ru | rv) < 0) {
-#pragma unroll
-        for (int b = 0; b < 8; ++b) steps += live[b] >= 0 ? 1u : 0u;
-        return false;
-      }
-      steps += 8;
+      steps += live;  // the walk ended without a boundary cell
+      return false;
     }
   } else {
     for (;;) {
@@ -443,7 +466,8 @@ __device__ inline bool line_clear(int x, int y, int rx, int ry, const BitView &b
 }
 
 __device__ inline BitView scen_bits(const Scen &sc, unsigned lds_off) {
-  return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.bw, sc.bh};
+  return BitView{sc.blk, lds_off, static_cast<int>(sc.g.W), sc.bw, sc.bh,
+                 static_cast<int>(sc.g.W * sc.g.H)};
 }
 
 
