@@ -453,6 +453,91 @@ __device__ bool walk_blocked(int fx, int fy, int tx, int ty, const BitView &bv, 
   }
 }
 
+// walk_blocked over the blocked bitset in global memory, for the deferred
+// walks: each batch of eight cells loads its 2 x 2 blocks once and spreads
+// them over a 16 x 16 window of this lane's LDS (`win`, 32 bytes; row r of
+// the window is bits 16 (r & 1) .. +15 of word r >> 1), so each cell is one
+// LDS read at window bit 16 ly + lx, as in the LDS form.  The batch's cells,
+// also the up to seven past the walk's end, lie within 7 cells of its first
+// cell in the walk's direction, so the window from block (xb, yb) =
+// floor(min(x0, x0 + 7 sx) / 8), floor(min(y0, y0 + 7 sy) / 8) holds them
+// all; the loads are clamped into the block grid, which changes only window
+// cells off the raster (past the walk's end, dropped by the dead mask).
+__device__ bool walk_window(int fx, int fy, int tx, int ty, const BitView &bv, uint32_t *win, uint32_t &steps) {
+  const int ddx = tx - fx, ddy = ty - fy;
+  int ru = abs(ddx), rv = abs(ddy);
+  const int sx = (ddx > 0) - (ddx < 0), sy = (ddy > 0) - (ddy < 0);
+  const float dlx = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(ru)));
+  const float dly = static_cast<float>(1.0 / static_cast<double>(static_cast<float>(rv)));
+  float tmx = dlx, tmy = dly;
+  const uint32_t wbits =
+      static_cast<uint32_t>(reinterpret_cast<size_t>((__attribute__((address_space(3))) uint32_t *)win)) << 3;
+  const int nsx = -sx, nsy = -16 * sy;
+  unsigned long long *w64 = reinterpret_cast<unsigned long long *>(win);
+  for (;;) {
+    const int x0 = tx + __mul24(nsx, ru), y0 = ty + __mul24(-sy, rv);
+    const int xb = min(x0, x0 + 7 * sx) >> 3, yb = min(y0, y0 + 7 * sy) >> 3;
+    const int bx0 = max(xb, 0), bx1 = min(xb + 1, bv.bw - 1);
+    const int by0 = max(yb, 0), by1 = min(yb + 1, bv.bh - 1);
+    const unsigned long long w00 = blk_at<false>(bv, static_cast<uint32_t>(bx0 + by0 * bv.bw));
+    const unsigned long long w10 = blk_at<false>(bv, static_cast<uint32_t>(bx1 + by0 * bv.bw));
+    const unsigned long long w01 = blk_at<false>(bv, static_cast<uint32_t>(bx0 + by1 * bv.bw));
+    const unsigned long long w11 = blk_at<false>(bv, static_cast<uint32_t>(bx1 + by1 * bv.bw));
+    // v_perm_b32 picks bytes of {S0, S1} (selector 0-3: S1's bytes, 4-7: S0's):
+    // rows 2j and 2j + 1 as (left, right, left, right) bytes.
+    const uint32_t a0 = static_cast<uint32_t>(w00), a1 = static_cast<uint32_t>(w00 >> 32);
+    const uint32_t b0 = static_cast<uint32_t>(w10), b1 = static_cast<uint32_t>(w10 >> 32);
+    const uint32_t c0 = static_cast<uint32_t>(w01), c1 = static_cast<uint32_t>(w01 >> 32);
+    const uint32_t e0 = static_cast<uint32_t>(w11), e1 = static_cast<uint32_t>(w11 >> 32);
+    constexpr uint32_t kLo = 0x05010400u, kHi = 0x07030602u;
+    w64[0] = __builtin_amdgcn_perm(b0, a0, kLo) | (static_cast<unsigned long long>(__builtin_amdgcn_perm(b0, a0, kHi)) << 32);
+    w64[1] = __builtin_amdgcn_perm(b1, a1, kLo) | (static_cast<unsigned long long>(__builtin_amdgcn_perm(b1, a1, kHi)) << 32);
+    w64[2] = __builtin_amdgcn_perm(e0, c0, kLo) | (static_cast<unsigned long long>(__builtin_amdgcn_perm(e0, c0, kHi)) << 32);
+    w64[3] = __builtin_amdgcn_perm(e1, c1, kLo) | (static_cast<unsigned long long>(__builtin_amdgcn_perm(e1, c1, kHi)) << 32);
+    // The cells are read by absolute LDS address (lds_bit_word), which the
+    // compiler cannot relate to `win`: keep the window's stores before these
+    // reads and the next batch's stores after them.
+    asm volatile("" ::: "memory");
+    // window bit of the cell with ru, rv steps left: 16 (y - 8 yb) + (x - 8 xb)
+    const int kb = static_cast<int>(wbits) + (tx - 8 * xb) + 16 * (ty - 8 * yb);
+    uint32_t id[8];
+    uint32_t dead = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      id[b] = static_cast<uint32_t>(kb + __mul24(nsx, ru) + __mul24(nsy, rv));
+      dead = __builtin_amdgcn_alignbit(dead, static_cast<uint32_t>(ru | rv), 31);
+      const float d = tmx - tmy;
+      const bool xs = d <= kTie, ys = !(d < -kTie);
+      ru -= xs ? 1 : 0;
+      rv -= ys ? 1 : 0;
+      tmx = xs ? tmx + dlx : tmx;
+      tmy = ys ? tmy + dly : tmy;
+    }
+    uint32_t acc = 0, wd[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      wd[b] = lds_bit_word(id[b]) >> (id[b] & 31u);
+      acc |= wd[b];
+    }
+    asm volatile("" ::: "memory");
+    if (!(acc & 1u) && (ru | rv) >= 0) {
+      steps += 8;
+      continue;
+    }
+    const uint32_t live = 8u - static_cast<uint32_t>(__builtin_popcount(dead & 0xffu));
+    uint32_t hits = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) hits |= (wd[b] & 1u) << b;
+    hits &= (1u << live) - 1u;
+    if (hits) {
+      steps += static_cast<uint32_t>(__builtin_ctz(hits)) + 1u;
+      return true;
+    }
+    steps += live;  // the walk ended without a boundary cell
+    return false;
+  }
+}
+
 // Line-of-sight flag for a committed reference: the DDA from the cell to an
 // adjacent reference visits exactly those two cells.
 template <bool kLdsBits>
@@ -570,6 +655,7 @@ static_assert(kEntryQ >= kWave, "one round of marks fits after at most one flush
 struct DeferLds {  // one wave's deferred work (LDS)
   WalkReq walk[kWalkQ];
   unsigned long long entry[kEntryQ];
+  uint32_t win[kWave][8];  // per lane: the 16 x 16 bitset window of a walk batch
 };
 
 template <bool kCache, bool kLdsBits>
@@ -580,6 +666,7 @@ struct WalkQueue {
   unsigned n;  // queued walks (wave-uniform)
   unsigned long long *e;  // this wave's kEntryQ list entries for the next sweep (LDS)
   unsigned ne;            // held entries (wave-uniform)
+  uint32_t (*win)[8];     // this wave's walk windows, one per lane (LDS)
   uint32_t changed_si = 0xFFFFFFFFu;  // the scenario whose last_change this wave set in this sweep
   unsigned long long *lc = nullptr;  // the workgroup's (scenario, sweep) pairs stored (LDS), or null
   uint32_t lc_mask = 0;              // its entries - 1 (direct-mapped by scenario)
@@ -590,8 +677,7 @@ template <bool kLdsBits>
 constexpr int last_change_entries() { return kLdsBits ? 16 : 256; }
 
 // Runs the queued walks, one per lane, and writes those cells' states.
-__device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
-  if (wq.n == 0) return;
+__device__ inline void run_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
   wave_sync();
   const unsigned lane = threadIdx.x & (kWave - 1);
   if (lane < wq.n) {
@@ -599,12 +685,27 @@ __device__ void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) 
     const Scen &sc = a.scen[e.si];
     const int x = static_cast<int>(e.cell & 0xffffu), y = static_cast<int>(e.cell >> 16);
     uint32_t ref = e.ref;
-    if (line_clear<false>(x, y, ref_x<true>(ref), ref_y<true>(ref), scen_bits(sc, 0), cnt.steps)) ref |= kClearBit;
+    const int rx = ref_x<true>(ref), ry = ref_y<true>(ref);
+    const BitView bv = scen_bits(sc, 0);
+    bool clear;
+    if (rx == x && ry == y) {
+      clear = false;  // an ignition's own reference: never walked
+    } else if (abs(rx - x) <= 1 && abs(ry - y) <= 1) {
+      cnt.steps += 2;
+      clear = !bit_at<false>(bv, x, y) && !bit_at<false>(bv, rx, ry);
+    } else {
+      clear = !walk_window(x, y, rx, ry, bv, wq.win[lane], cnt.steps);
+    }
+    if (clear) ref |= kClearBit;
     uint2 *st_out = (k & 1) ? sc.st[0] : sc.st[1];
     st_state(&st_out[static_cast<size_t>(x) + static_cast<size_t>(y) * sc.g.W], pack_state(e.time, ref));
   }
   wq.n = 0;
   wave_sync();
+}
+
+__device__ inline void flush_walks(const Args &a, int k, WalkQueue &wq, Counters &cnt) {
+  if (wq.n != 0) run_walks(a, k, wq, cnt);
 }
 
 constexpr unsigned long long kNoEntry = ~0ull;
@@ -1343,7 +1444,7 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
 template <bool kCache, bool kLdsBits, int TW>
 __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, int k, TileLds<TW> &s, DeferLds *dq,
                                Counters &cnt, unsigned long long *last_change) {
-  WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
+  WalkQueue wq{dq->walk, 0u, dq->entry, 0u, dq->win};
   wq.lc = last_change;
   wq.lc_mask = static_cast<uint32_t>(last_change_entries<kLdsBits>() - 1);
   const unsigned lane = threadIdx.x & (kWave - 1);
@@ -1394,7 +1495,7 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
 template <bool kCache, bool kLdsBits, int TW>
 __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsigned lds_off, int k, TileLds<TW> &s,
                            DeferLds *dq, Counters &cnt) {
-  WalkQueue wq{dq->walk, 0u, dq->entry, 0u};
+  WalkQueue wq{dq->walk, 0u, dq->entry, 0u, dq->win};
   const unsigned lane = threadIdx.x & (kWave - 1);
   bool changed = false;
   for (uint32_t x = 0; x < a.q.nsub; ++x) {
