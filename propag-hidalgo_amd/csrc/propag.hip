@@ -219,14 +219,32 @@ __device__ inline void sincos_det(double x, double &s, double &c) {
   }
 }
 
-// time_from (propag.cu:269-294) in closed form; (fx,fy) -> (tx,ty), fire k.
+// sqrtf(x), correctly rounded, for x in [1, 2^126): the hardware root s
+// (within one ulp) corrected by a Tuckerman-style rounding test, the
+// products of s with its neighbours s -+ ulp standing in for the squared
+// midpoints, each residual one fused multiply-add: the root is s - ulp when
+// x <= (s - ulp) s, s + ulp when x > (s + ulp) s, else s.  Checked on the host against the correctly rounded root
+// for every float in [1, 4) and every s within one ulp of it (x -> 4x scales
+// s by 2 exactly).  No scaling for tiny x and no special values, which
+// sqrtf's general expansion spends about half its instructions on.
+__device__ inline float sqrt_rn_ge1(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const uint32_t b = __float_as_uint(s);
+  const float sd = __uint_as_float(b - 1u), su = __uint_as_float(b + 1u);
+  float r = fmaf(-sd, s, x) <= 0.0f ? sd : s;
+  r = fmaf(-su, s, x) > 0.0f ? su : r;
+  return r;
+}
+
+// time_from (propag.cu:269-294) in closed form; (fx,fy) -> (tx,ty), fire k,
+// (fx,fy) != (tx,ty).
 __device__ inline float time_from(float from_t, int fx, int fy, int tx, int ty, const float4 &k,
                                   const Geom &g) {
   const int ix = tx - fx;
   const int iy = ty - fy;
   const float ux = static_cast<float>(ix);
   const float uy = static_cast<float>(g.north_up ? -iy : iy);
-  const float r = sqrtf(ux * ux + uy * uy);
+  const float r = sqrt_rn_ge1(ux * ux + uy * uy);  // integer cells apart: ux^2 + uy^2 >= 1
   const float proj = k.x * uy + k.y * ux;
   if (g.square) return from_t + ((r - proj) * g.cell) * k.z;
   const float ex = ux * g.dx;
@@ -991,7 +1009,9 @@ __device__ inline float from_order_bits(uint32_t o) {
 // changed in sweep k - 1 (chg bit 0, stamps) that can feed the rule (bit 1)
 // are evaluated: their (cell, neighbour) pairs are packed over the wave's
 // lanes and each cell's least key (order-mapped time, neighbour index,
-// reference-or-neighbour bit) is an LDS atomic minimum.
+// reference-or-neighbour bit) is an LDS atomic minimum.  Neighbour (i, j)
+// has the index (i + 1) + 4 (j + 1): ordered as the reference's loops (j
+// outer, i inner) and decoded with a mask and a shift.
 template <int TW>
 __device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy, int lx, int ly, int gx, int gy,
                                    int lane, bool need, Counters &cnt, bool &improved, float &best_t_out,
@@ -1006,7 +1026,7 @@ __device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy
       const int i = d % 3 - 1, j = d / 3 - 1;
       const int nx = gx + i, ny = gy + j;
       const bool inb = nx >= 0 && ny >= 0 && nx < g.W && ny < g.H;
-      m |= (inb && s.chg[(lx + i) + (ly + j) * HW] == 3) ? (1u << d) : 0u;
+      m |= (inb && s.chg[(lx + i) + (ly + j) * HW] == 3) ? (1u << ((i + 1) + 4 * (j + 1))) : 0u;
     }
   }
   // Exclusive wave scan of the pair counts (0..8), one ballot per bit.
@@ -1030,7 +1050,7 @@ __device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy
       const int cl = static_cast<int>(pr >> 4), d = static_cast<int>(pr & 15u);
       const int cx = cl % TW + 1, cy = cl / TW + 1;
       const int tx = ox + cx - 1, ty = oy + cy - 1;
-      const int i = d % 3 - 1, j = d / 3 - 1;
+      const int i = (d & 3) - 1, j = (d >> 2) - 1;
       const int nl = (cx + i) + (cy + j) * HW;
       const uint32_t ref = s.ref[nl];
       const int rfx = ref_x<true>(ref), rfy = ref_y<true>(ref);
@@ -1055,7 +1075,7 @@ __device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy
     improved = key != ~0ull && cand < own_t;  // update_point (propag.cu:192-200)
     if (improved) {
       const int d = static_cast<int>((key >> 1) & 15u);
-      const int i = d % 3 - 1, j = d / 3 - 1;
+      const int i = (d & 3) - 1, j = (d >> 2) - 1;
       const uint32_t ref = s.ref[(lx + i) + (ly + j) * HW];
       best_t_out = cand;
       best_ref_out = (key & 1u) ? make_ref(ref_x<true>(ref), ref_y<true>(ref)) : make_ref(gx + i, gy + j);
