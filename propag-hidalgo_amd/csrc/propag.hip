@@ -641,7 +641,7 @@ constexpr int kProfCols = 8;
 struct Counters {
   uint32_t relax, improve, steps, tiles, marks, idle;
   uint32_t idle_far;                // profiling: idle visits with no change within two cells
-  uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (thread 0 of each workgroup)
+  uint64_t work_ticks, wait_ticks;  // wall_clock64 ticks (wave 0 of each workgroup)
   uint64_t crit_ticks;              // workgroup 0: slowest workgroup's work per sweep, summed
   uint64_t ph[4];                   // profiling: this wave's phase ticks in the sweep
   uint64_t ph_total[4];             // profiling: this wave's phase ticks, all sweeps
@@ -1018,8 +1018,8 @@ __device__ inline void packed_rule(TileLds<TW> &s, const Geom &g, int ox, int oy
                                    uint32_t &best_ref_out) {
   constexpr int HW = Tile<TW>::HW;
   uint32_t m = 0;
+  cnt.relax += static_cast<uint32_t>(__builtin_popcountll(__ballot(need)));
   if (need) {
-    ++cnt.relax;
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
       if (d == 4) continue;
@@ -1191,8 +1191,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   uint32_t best_ref_out = 0u;
   if constexpr (kCache) {
     packed_rule<TW>(s, g, ox, oy, lx, ly, gx, gy, lane, need, cnt, improved, best_t_out, best_ref_out);
-  } else if (need) {
-    ++cnt.relax;
+  } else if (cnt.relax += static_cast<uint32_t>(__builtin_popcountll(__ballot(need))); need) {
     const int me = lx + ly * HW;
     const float own_t = s.time[me];
     float best_t = own_t;
@@ -1243,8 +1242,9 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
   // round trip overlaps the line-of-sight walks below.
   const unsigned long long imp = __ballot(improved);
   const bool changed = imp != 0ull;
+  cnt.improve += static_cast<uint32_t>(__builtin_popcountll(imp));
   const bool idle = __ballot(need) == 0ull;  // (a ballot of every lane)
-  if (lane == 0 && idle) ++cnt.idle;
+  if (idle) ++cnt.idle;
   if (TW == kTile && a.prof && idle) {
     // Diagnostics: nothing in an idle tile's halo changed; if nothing two
     // cells from the tile changed either, only a far-dependents box queued
@@ -1265,7 +1265,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       if (rgx >= 0 && rgy >= 0 && rgx < g.W && rgy < g.H)
         near = ld_i32(&stamp[static_cast<size_t>(rgx) + static_cast<size_t>(rgy) * g.W]) >= k - 1;
     }
-    if (__ballot(near) == 0ull && lane == 0) ++cnt.idle_far;
+    if (__ballot(near) == 0ull) ++cnt.idle_far;
   }
   NextTiles nt{0, 0, 1, 0};
   Mark mk0{0u, 0u, 0u, false};
@@ -1298,14 +1298,15 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
         else if (line_clear<kLdsBits>(gx, gy, ref_x<true>(best_ref), ref_y<true>(best_ref), bv, cnt.steps))
           best_ref |= kClearBit;
       }
-      if (a.prof) walk_ticks += wall_clock64() - tw;
+      if constexpr (!kDefer) {
+        if (a.prof) walk_ticks += wall_clock64() - tw;
+      }
       const int brx = ref_x<kCache>(best_ref), bry = ref_y<kCache>(best_ref);
       far_dep = !same_ref && (abs(brx - gx) > 1 || abs(bry - gy) > 1);
       rtile = static_cast<uint32_t>(brx / TW) + static_cast<uint32_t>(bry / TH) * static_cast<uint32_t>(sc.ntx);
       rbit = static_cast<uint32_t>(brx % TW + (bry % TH) * TW);
       if (!deferred) st_state(&st_out[gi], pack_state(best_t, best_ref));
       st_i32(&stamp[gi], k);
-      ++cnt.improve;
     } else {
       st_state(&st_out[gi], pack_state(s.time[me], s.ref[me]));
     }
@@ -1334,7 +1335,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     else
       mark_commit(a.q, si, k + 1, mk0, lane);
     const uint32_t marks = next_rest<kDefer, TW>(a.q, sc, si, k, nt, wq, lane);
-    if (lane == 0) cnt.marks += marks;
+    cnt.marks += __builtin_amdgcn_readfirstlane(marks);
     // Register this tile as a dependent of each new far reference's tile
     // and mark the reference cells: each distinct reference tile's group is
     // gathered in its first lane, which then reads that tile's box and
@@ -1395,7 +1396,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       if (g_box.w > cur[3]) max_relaxed(b + 3, g_box.w);
     }
   }
-  if (lane == 0) ++cnt.tiles;
+  ++cnt.tiles;
   wave_sync();  // the next tile reuses this wave's LDS slice
   if (a.prof) {
     // per wave: staging, rule (minus walks), walks (slowest lane), tail
@@ -1565,10 +1566,12 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
   Control *c = a.ctl;
   unsigned long long *arrive = a.arrive + (target & 1u) * static_cast<size_t>(a.arrive_stride);
   uint64_t t_arrive = 0;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kWave) {  // wave 0 keeps the workgroup's times (wave-uniform)
     t_arrive = wall_clock64();
+    cnt.work_ticks += t_arrive - sweep_t0;
+  }
+  if (threadIdx.x == 0) {
     const uint64_t work = t_arrive - sweep_t0;
-    cnt.work_ticks += work;
     if (a.prof) {
       const unsigned par = target & 1u;
       for (int q = 0; q < 4; ++q) st_u64(&slots[blockIdx.x].ph[par][q], wg_ph[q]);
@@ -1653,12 +1656,12 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
           r[7] = t_all;  // completed below with the end of the books
         }
       }
+      cnt.crit_ticks += maxwork;  // (reduced over the wave) written out once, at the end of the launch
       if (lane == 0) {
         if (done) {
           st_relaxed(&c->book.v[4], static_cast<unsigned>(k));
           st_relaxed(&c->book.v[1], changed ? 0u : 1u);
         }
-        cnt.crit_ticks += maxwork;  // written out once, at the end of the launch
         // Sweep k + 2 appends to list (k + 3) & 3, last read in sweep k - 1;
         // these stores complete before this wave's next barrier arrival.
         for (uint32_t x = 0; x < a.q.nsub; ++x) st_relaxed(count_of(a.q, k + 3, x), 0u);
@@ -1677,22 +1680,24 @@ __device__ int grid_barrier(const Args &a, int k, bool wg_changed, uint64_t swee
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) cnt.wait_ticks += wall_clock64() - t_arrive;
+  if (threadIdx.x < kWave) cnt.wait_ticks += wall_clock64() - t_arrive;
   return s_done;
 }
 
 __device__ inline void flush_counters(const Args &a, const Counters &cnt) {
-  // One 64-bit add per counter per wave.
-  unsigned long long v[16] = {cnt.relax, cnt.improve, cnt.steps, cnt.tiles, cnt.work_ticks,
-                              cnt.wait_ticks, cnt.crit_ticks, cnt.marks, 0ull, 0ull, 0ull, 0ull, cnt.idle, cnt.idle_far, 0ull, 0ull};
-  // phase totals are per wave (lane 0's copy only)
-  const bool lead = (threadIdx.x & 63) == 0;
-  for (int q = 0; q < 4; ++q) v[8 + q] = lead ? cnt.ph_total[q] : 0ull;
+  // The counters are wave-uniform (kept in scalar registers), except the
+  // walk steps, which each lane counts for its own walks.  One 64-bit add
+  // per counter per wave.
+  unsigned long long steps = cnt.steps;
+  for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
+  const unsigned long long v[16] = {cnt.relax,      cnt.improve,      steps,            cnt.tiles,
+                                    cnt.work_ticks, cnt.wait_ticks,   cnt.crit_ticks,   cnt.marks,
+                                    cnt.ph_total[0], cnt.ph_total[1], cnt.ph_total[2], cnt.ph_total[3],
+                                    cnt.idle,       cnt.idle_far,     0ull,             0ull};
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    unsigned long long x = v[q];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    if ((threadIdx.x & 63) == 0 && x) atomicAdd(&a.ctl->stats[q], x);
+    for (int q = 0; q < 16; ++q)
+      if (v[q]) atomicAdd(&a.ctl->stats[q], v[q]);
   }
 }
 
@@ -1736,14 +1741,14 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   }
   __syncthreads();
   Counters cnt{};
-  uint64_t t0 = threadIdx.x == 0 ? wall_clock64() : 0;
+  uint64_t t0 = wall_clock64();
   for (int k = a.k_begin;; ++k) {
     const bool changed = sweep_wave_dyn<kCache, kLdsBits, TW>(a, s_next[k & 1], lds_off, k, s, dq, cnt, s_lastc);
     if (lane == 0 && changed) s_changed[k & 1] = 1u;
-    if (a.prof && lane == 0) {
+    if (a.prof) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        atomicMax(&s_ph[k & 1][q], static_cast<unsigned long long>(cnt.ph[q]));
+        if (lane == 0) atomicMax(&s_ph[k & 1][q], static_cast<unsigned long long>(cnt.ph[q]));
         cnt.ph_total[q] += cnt.ph[q];
         cnt.ph[q] = 0;
       }
@@ -1751,10 +1756,8 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
     wait_stores();
     __syncthreads();
     const int done = grid_barrier(a, k, s_changed[k & 1] != 0u, t0, cnt, s_ph[k & 1]);
-    if (threadIdx.x == 0) {
-      s_changed[k & 1] = 0;
-      t0 = wall_clock64();
-    }
+    if (threadIdx.x == 0) s_changed[k & 1] = 0;
+    t0 = wall_clock64();
     if (threadIdx.x < kXcds) s_next[k & 1][threadIdx.x] = 0;
     if (threadIdx.x < 4) s_ph[k & 1][threadIdx.x] = 0;
     if (done) break;
