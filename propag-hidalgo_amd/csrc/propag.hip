@@ -1425,7 +1425,19 @@ __device__ inline bool run_entry(const Args &a, unsigned long long ev, unsigned 
   const uint32_t si = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev >> 32));
   const uint32_t tile = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(ev) & kTileMask);
   if (si >= static_cast<uint32_t>(a.nscen)) return false;
-  const Scen &sc = a.scen[si];
+  // The descriptors do not change during a launch, and si is wave-uniform:
+  // read through the constant address space, the fields come in scalar loads
+  // and stay in scalar registers instead of taking vector registers in every
+  // lane.
+  Scen sc;
+  {
+    static_assert(sizeof(Scen) % 4 == 0, "copied by words");
+    const __attribute__((address_space(4))) uint32_t *src =
+        reinterpret_cast<const __attribute__((address_space(4))) uint32_t *>(reinterpret_cast<uintptr_t>(a.scen + si));
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&sc);
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(Scen) / 4; ++i) dst[i] = src[i];
+  }
   if (sc.max_iters == 0u) return false;  // released scenario: its buffers are gone
   if (lane == 0) {
     const uint32_t bit = 1u << (tile >> sc.nwords_log2);
