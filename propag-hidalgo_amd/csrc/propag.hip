@@ -159,6 +159,17 @@ __device__ inline void st_state(uint2 *p, uint2 v) {
 // moving memory accesses across it.
 __device__ inline void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// Lane l's value for the whole wave, l the same in every lane (a constant
+// or a value read off a ballot): v_readlane into a scalar register, with no
+// LDS round trip as a shuffle takes.
+__device__ inline uint32_t lane_value(uint32_t v, int l) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
+}
+__device__ inline unsigned long long lane_value(unsigned long long v, int l) {
+  return static_cast<unsigned long long>(lane_value(static_cast<uint32_t>(v), l)) |
+         (static_cast<unsigned long long>(lane_value(static_cast<uint32_t>(v >> 32), l)) << 32);
+}
+
 __device__ inline void wave_sync() {
   // Lanes of one wave exchange data through LDS: order the accesses.
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -852,7 +863,7 @@ struct NextTiles {
 template <int TW>
 __device__ inline NextTiles next_tiles(const Scen &sc, uint32_t bx) {
   NextTiles nt{0, 0, 1, 0};
-  const uint32_t x0 = __shfl(bx, 0), y0 = __shfl(bx, 1), x1 = __shfl(bx, 2), y1 = __shfl(bx, 3);
+  const uint32_t x0 = lane_value(bx, 0), y0 = lane_value(bx, 1), x1 = lane_value(bx, 2), y1 = lane_value(bx, 3);
   if (x0 <= x1 && y0 <= y1) {  // an empty box is x0 = y0 = 0xFFFFFFFF, x1 = y1 = 0
     nt.X0 = max(static_cast<int>(x0) - 1, 0) / TW;
     nt.Y0 = max(static_cast<int>(y0) - 1, 0) / Tile<TW>::H;
@@ -1273,7 +1284,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     // The far dependents need the next sweep only if a changed cell is
     // some cell's far reference.
     const unsigned long long refs =
-        static_cast<unsigned long long>(__shfl(bx, 4)) | (static_cast<unsigned long long>(__shfl(bx, 5)) << 32);
+        static_cast<unsigned long long>(lane_value(bx, 4)) | (static_cast<unsigned long long>(lane_value(bx, 5)) << 32);
     if (imp & refs) nt = next_tiles<TW>(sc, bx);
     const uint32_t nb =
         kLdsBits ? near_band<TW>(imp) : next_neighbours<kCache, TW>(s, ox, oy, lx, ly, improved, imp, lane);
@@ -1351,7 +1362,7 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
     uint4 g_box = make_uint4(0u, 0u, 0u, 0u);
     while (m) {
       const int l = __builtin_ctzll(m);
-      const uint32_t rt0 = __shfl(rtile, l);
+      const uint32_t rt0 = lane_value(rtile, l);
       const bool grp = far_dep && rtile == rt0;
       const unsigned long long gm = __ballot(grp);
       m &= ~gm;
@@ -1495,7 +1506,7 @@ __device__ bool sweep_wave_dyn(const Args &a, unsigned *next, unsigned lds_off, 
     auto draw = [&](size_t &e) {
       unsigned i = 0;
       if (lane == 0) i = atomicAdd(&next[x], 1u);
-      i = __shfl(i, 0);
+      i = lane_value(i, 0);
       e = r + static_cast<size_t>(nr) * i;
       return e < a.q.cap ? ld_u64(&list[e]) : kNoEntry;
     };
@@ -1542,7 +1553,7 @@ __device__ bool sweep_wave(const Args &a, unsigned gwave, unsigned nwaves, unsig
       for (unsigned long long r = m; r;) {
         const int l = __builtin_ctzll(r);
         r &= r - 1;
-        changed |= run_entry<kCache, kLdsBits, TW>(a, __shfl(v, l), lds_off, k, s, wq, cnt);
+        changed |= run_entry<kCache, kLdsBits, TW>(a, lane_value(v, l), lds_off, k, s, wq, cnt);
       }
       if (m != ~0ull) break;
     }
