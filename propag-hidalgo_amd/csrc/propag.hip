@@ -170,6 +170,20 @@ __device__ inline unsigned long long lane_value(unsigned long long v, int l) {
          (static_cast<unsigned long long>(lane_value(static_cast<uint32_t>(v >> 32), l)) << 32);
 }
 
+// OR of v over the wave's 64 lanes, every lane active: prefix ORs within
+// each row of 16 lanes by DPP row shifts (row_shr 1, 2, 4, 8; a lane shifted
+// in from outside the row reads 0), after which lanes 15, 31, 47 and 63 hold
+// their rows' ORs.  VALU and scalar only, no LDS round trip.
+__device__ inline uint32_t wave_or(uint32_t v) {
+  int x = static_cast<int>(v);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);
+  x |= __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(x, 15) | __builtin_amdgcn_readlane(x, 31) |
+                               __builtin_amdgcn_readlane(x, 47) | __builtin_amdgcn_readlane(x, 63));
+}
+
 __device__ inline void wave_sync() {
   // Lanes of one wave exchange data through LDS: order the accesses.
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1366,11 +1380,8 @@ __device__ bool wave_tile(const Args &a, const Scen &sc, uint32_t si, unsigned l
       const bool grp = far_dep && rtile == rt0;
       const unsigned long long gm = __ballot(grp);
       m &= ~gm;
-      uint32_t lo = (grp && rbit < 32u) ? 1u << rbit : 0u, hi = (grp && rbit >= 32u) ? 1u << (rbit - 32u) : 0u;
-      for (int off = 32; off > 0; off >>= 1) {
-        lo |= __shfl_xor(lo, off);
-        hi |= __shfl_xor(hi, off);
-      }
+      const uint32_t lo = wave_or((grp && rbit < 32u) ? 1u << rbit : 0u);
+      const uint32_t hi = wave_or((grp && rbit >= 32u) ? 1u << (rbit - 32u) : 0u);
       if (lane == l) {
         lead = true;
         g_rt = rt0;
