@@ -1771,7 +1771,29 @@ __global__ __launch_bounds__(persistent_threads<kLdsBits>()) void propagate_pers
   if (threadIdx.x < 8) s_ph[threadIdx.x / 4][threadIdx.x % 4] = 0;
   for (unsigned i = threadIdx.x; i < static_cast<unsigned>(kLastChange); i += blockDim.x) s_lastc[i] = ~0ull;
   if (kLdsBits) {
-    for (unsigned i = threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
+    // 16-byte chunks, eight loads in flight per thread before their LDS
+    // writes (one dependent round trip per word otherwise), then the tail.
+    unsigned done = 0;
+    if ((reinterpret_cast<uintptr_t>(a.lds_src) & 15u) == 0u) {
+      const unsigned n4 = a.lds_words / 4u;
+      const uint4 *src4 = reinterpret_cast<const uint4 *>(a.lds_src);
+      uint4 *dst4 = reinterpret_cast<uint4 *>(dyn_lds);
+      for (unsigned i0 = threadIdx.x; i0 < n4; i0 += 8u * blockDim.x) {
+        uint4 v[8];
+#pragma unroll
+        for (unsigned j = 0; j < 8u; ++j) {
+          const unsigned i = i0 + j * blockDim.x;
+          if (i < n4) v[j] = src4[i];
+        }
+#pragma unroll
+        for (unsigned j = 0; j < 8u; ++j) {
+          const unsigned i = i0 + j * blockDim.x;
+          if (i < n4) dst4[i] = v[j];
+        }
+      }
+      done = n4 * 4u;
+    }
+    for (unsigned i = done + threadIdx.x; i < a.lds_words; i += blockDim.x) dyn_lds[i] = a.lds_src[i];
   }
   __syncthreads();
   Counters cnt{};
